@@ -1,0 +1,151 @@
+// C-ABI operator applies: argument checks, kernel lookup and launch.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "gd_dispatch.hpp"
+
+namespace gd {
+
+const KernelSet* find_kernels(int D, int N, int Q)
+{
+  KernelList l{nullptr, 0};
+  switch (D) {
+    case 1: l = kernels_d1(); break;
+    case 2: l = kernels_d2(); break;
+    case 3: l = kernels_d3(); break;
+    case 4: l = kernels_d4(); break;
+    case 6: l = kernels_d6(); break;
+    default: break;
+  }
+  for (int i = 0; i < l.count; ++i)
+    if (l.sets[i].N == N && l.sets[i].Q == Q) return &l.sets[i];
+  return nullptr;
+}
+
+namespace {
+
+int lookup(const gd_plan* plan, const KernelSet** ks)
+{
+  if (!plan) return fail(GD_ERR_INVALID, "null plan");
+  if (plan->padapt) return fail(GD_ERR_UNSUPPORTED, "p-adaptive plans use gd_padapt_apply_*");
+  const int N = plan->desc.order + 1, Q = plan->desc.num_quad;
+  *ks = find_kernels(plan->dim, N, Q);
+  if (!*ks)
+    return fail(GD_ERR_UNSUPPORTED, "no kernel compiled for dim=" + std::to_string(plan->dim) +
+                                        " nodes=" + std::to_string(N) + " quad=" + std::to_string(Q));
+  return GD_OK;
+}
+
+int check_adv(const gd_plan* plan, bool need_pointwise)
+{
+  if (!plan->beta_set) return fail(GD_ERR_INVALID, "advection field not set");
+  if (need_pointwise && plan->beta_kind != GD_BETA_POINTWISE)
+    return fail(GD_ERR_INVALID, "gradient w.r.t. beta needs pointwise beta");
+  return GD_OK;
+}
+
+template <typename T>
+int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, hipStream_t s)
+{
+  const KernelSet* ks = nullptr;
+  int rc = lookup(plan, &ks);
+  if (rc) return rc;
+  if (!x || !y) return fail(GD_ERR_INVALID, "null vector");
+  if ((const void*)x == (const void*)y) return fail(GD_ERR_INVALID, "input and output must not alias");
+  const bool adv = adv_coef != T(0);
+  if (adv) {
+    rc = check_adv(plan, false);
+    if (rc) return rc;
+  }
+  if constexpr (std::is_same_v<T, float>) {
+    if (adv && plan->beta_kind == GD_BETA_POINTWISE)
+      return fail(GD_ERR_UNSUPPORTED, "pointwise beta is FP64 only");
+  }
+  const TableSet* ts = find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
+  const Params<T> prm = make_params<T>(plan, ts, mass_coef, adv_coef);
+  if constexpr (std::is_same_v<T, double>) return adv ? ks->fwd64_adv(prm, x, y, s) : ks->fwd64_mass(prm, x, y, s);
+  else return adv ? ks->fwd32_adv(prm, x, y, s) : ks->fwd32_mass(prm, x, y, s);
+}
+
+}  // namespace
+}  // namespace gd
+
+using gd::fail;
+
+extern "C" {
+
+int gd_apply_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* x, double* y,
+                 gd_stream stream)
+{
+  return gd::apply_impl<double>(plan, mass_coef, adv_coef, x, y, (hipStream_t)stream);
+}
+
+int gd_apply_f32(const gd_plan* plan, float mass_coef, float adv_coef, const float* x, float* y,
+                 gd_stream stream)
+{
+  return gd::apply_impl<float>(plan, mass_coef, adv_coef, x, y, (hipStream_t)stream);
+}
+
+int gd_mass_apply_f64(const gd_plan* plan, const double* x, double* y, gd_stream stream)
+{
+  return gd_apply_f64(plan, 1.0, 0.0, x, y, stream);
+}
+
+int gd_mass_apply_f32(const gd_plan* plan, const float* x, float* y, gd_stream stream)
+{
+  return gd_apply_f32(plan, 1.0f, 0.0f, x, y, stream);
+}
+
+int gd_adv_apply_f64(const gd_plan* plan, const double* u, double* y, gd_stream stream)
+{
+  return gd_apply_f64(plan, 0.0, 1.0, u, y, stream);
+}
+
+int gd_adv_apply_f32(const gd_plan* plan, const float* u, float* y, gd_stream stream)
+{
+  return gd_apply_f32(plan, 0.0f, 1.0f, u, y, stream);
+}
+
+int gd_apply_T_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* r, double* z,
+                   gd_stream stream)
+{
+  const gd::KernelSet* ks = nullptr;
+  int rc = gd::lookup(plan, &ks);
+  if (rc) return rc;
+  if (!r || !z) return fail(GD_ERR_INVALID, "null vector");
+  if ((const void*)r == (const void*)z) return fail(GD_ERR_INVALID, "input and output must not alias");
+  const gd::TableSet* ts = gd::find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
+  const gd::Params<double> prm = gd::make_params<double>(plan, ts, mass_coef, adv_coef);
+  if (adv_coef == 0.0) return ks->fwd64_mass(prm, r, z, (hipStream_t)stream);  // M is symmetric
+  rc = gd::check_adv(plan, false);
+  if (rc) return rc;
+  return ks->trn64(prm, r, z, (hipStream_t)stream);
+}
+
+int gd_adv_apply_T_f64(const gd_plan* plan, const double* r, double* z, gd_stream stream)
+{
+  return gd_apply_T_f64(plan, 0.0, 1.0, r, z, stream);
+}
+
+int gd_adv_grad_beta_f64(const gd_plan* plan, double adv_coef, const double* u, const double* r,
+                         double* dbeta_cell, double* const* dbeta_face, gd_stream stream)
+{
+  const gd::KernelSet* ks = nullptr;
+  int rc = gd::lookup(plan, &ks);
+  if (rc) return rc;
+  rc = gd::check_adv(plan, true);
+  if (rc) return rc;
+  if (!u || !r || !dbeta_cell || !dbeta_face) return fail(GD_ERR_INVALID, "null argument");
+  const gd::TableSet* ts = gd::find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
+  gd::Params<double> prm = gd::make_params<double>(plan, ts, 0.0, adv_coef);
+  prm.dbeta_cell = dbeta_cell;
+  for (int d = 0; d < plan->dim; ++d) {
+    if (!dbeta_face[d]) return fail(GD_ERR_INVALID, "null face gradient array");
+    prm.dbeta_face[d] = dbeta_face[d];
+  }
+  // the kernel reads r through its second vector argument
+  return ks->grad64(prm, u, const_cast<double*>(r), (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,208 @@
+"""Operators with GenDiL's call shape, backed by the gfx950 kernels.
+
+  MakeMassFiniteElementOperator(fes, ir, sigma)       massoperator.hpp:270
+  MakeAdvectionOperator(fes, ir, adv[, bc])           advectionoperator.hpp:1416-1528
+  op(x, y)  overwrites y (L2 semantics, massoperator.hpp:209-216,
+            advectionoperator.hpp:1070-1076)
+
+Vectors are torch tensors on the GPU (float64 or float32); the apply is
+launched on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .discretization import FiniteElementSpace, IntegrationRule, make_desc
+from .fields import AffineField, PointwiseField, SeparableField, VectorField
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check_vec(t, n, dtype, name):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA (HIP) torch tensor")
+    if t.dtype != dtype or t.numel() != n or not t.is_contiguous():
+        raise ValueError(f"{name}: expected contiguous {dtype} with {n} entries, got {t.dtype} {tuple(t.shape)}")
+
+
+class Plan:
+    """Owns one gd_plan: mesh, basis tables and coefficients on the device."""
+
+    def __init__(self, fes: FiniteElementSpace, ir: Optional[IntegrationRule]):
+        self.fes = fes
+        self.ir = ir
+        self.q = ir.num_points if ir is not None else fes.fe.order + 1
+        desc, keep = make_desc(fes, ir)
+        h = C.c_void_p()
+        L.check(L.lib().gd_plan_create(C.byref(desc), C.byref(h)), "gd_plan_create")
+        self.handle = h
+        self._keep = [keep]
+        self.num_dofs = int(L.lib().gd_plan_num_dofs(self.handle))
+        self.beta_kind = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                L.lib().gd_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    # ---- coefficients ----
+    def set_sigma(self, sigma):
+        if sigma is None:
+            return
+        if isinstance(sigma, (int, float)):
+            sigma = SeparableField.constant(float(sigma))
+        s, data = sigma.tabulate(self.fes.mesh, self.q)
+        L.check(L.lib().gd_plan_set_sigma(self.handle, C.byref(s)), "gd_plan_set_sigma")
+
+    def set_beta(self, adv):
+        D = self.fes.dim
+        if isinstance(adv, VectorField):
+            if len(adv.components) != D:
+                raise ValueError("one component per dimension")
+            arr = (L.GdSeparable * D)()
+            keep = []
+            for d, comp in enumerate(adv.components):
+                s, data = comp.tabulate(self.fes.mesh, self.q)
+                arr[d] = s
+                keep.append(data)
+            L.check(L.lib().gd_plan_set_beta_separable(self.handle, arr), "gd_plan_set_beta_separable")
+            self.beta_kind = L.BETA_SEPARABLE
+        elif isinstance(adv, AffineField):
+            A = np.ascontiguousarray(np.asarray(adv.A, dtype=np.float64).reshape(D, D))
+            c = np.ascontiguousarray(np.asarray(adv.c, dtype=np.float64).reshape(D))
+            L.check(
+                L.lib().gd_plan_set_beta_affine(self.handle, A.ctypes.data_as(L._pd), c.ctypes.data_as(L._pd)),
+                "gd_plan_set_beta_affine",
+            )
+            self.beta_kind = L.BETA_AFFINE
+        elif isinstance(adv, PointwiseField):
+            self.set_beta_pointwise(adv.cell, adv.faces)
+        else:
+            raise TypeError("adv must be VectorField, AffineField or PointwiseField")
+
+    def set_beta_pointwise(self, cell: torch.Tensor, faces: Sequence[torch.Tensor]):
+        D = self.fes.dim
+        for t in [cell, *faces]:
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError("pointwise beta must be contiguous float64 device tensors")
+        arr = (C.c_void_p * D)(*[f.data_ptr() for f in faces])
+        L.check(L.lib().gd_plan_set_beta_pointwise_f64(self.handle, cell.data_ptr(), arr), "set_beta_pointwise")
+        self._beta_pw = (cell, list(faces))
+        self.beta_kind = L.BETA_POINTWISE
+
+    def set_bc(self, bc):
+        if bc is None:
+            L.check(L.lib().gd_plan_set_bc(self.handle, L.BC_NONE, None), "gd_plan_set_bc")
+            return
+        if isinstance(bc, (int, float)):
+            bc = SeparableField.constant(float(bc))
+        s, data = bc.tabulate(self.fes.mesh, self.q)
+        L.check(L.lib().gd_plan_set_bc(self.handle, L.BC_INFLOW, C.byref(s)), "gd_plan_set_bc")
+
+    # ---- applies ----
+    def apply(self, mass_coef: float, adv_coef: float, x: torch.Tensor, y: torch.Tensor):
+        if x.dtype == torch.float64:
+            _check_vec(x, self.num_dofs, torch.float64, "x")
+            _check_vec(y, self.num_dofs, torch.float64, "y")
+            L.check(L.lib().gd_apply_f64(self.handle, mass_coef, adv_coef, x.data_ptr(), y.data_ptr(), _stream()),
+                    "gd_apply_f64")
+        elif x.dtype == torch.float32:
+            _check_vec(x, self.num_dofs, torch.float32, "x")
+            _check_vec(y, self.num_dofs, torch.float32, "y")
+            L.check(L.lib().gd_apply_f32(self.handle, mass_coef, adv_coef, x.data_ptr(), y.data_ptr(), _stream()),
+                    "gd_apply_f32")
+        else:
+            raise TypeError("x must be float64 or float32")
+        return y
+
+    def apply_transpose(self, mass_coef: float, adv_coef: float, r: torch.Tensor, z: torch.Tensor):
+        _check_vec(r, self.num_dofs, torch.float64, "r")
+        _check_vec(z, self.num_dofs, torch.float64, "z")
+        L.check(L.lib().gd_apply_T_f64(self.handle, mass_coef, adv_coef, r.data_ptr(), z.data_ptr(), _stream()),
+                "gd_apply_T_f64")
+        return z
+
+    def grad_beta(self, adv_coef: float, u: torch.Tensor, r: torch.Tensor, dcell: torch.Tensor, dfaces):
+        _check_vec(u, self.num_dofs, torch.float64, "u")
+        _check_vec(r, self.num_dofs, torch.float64, "r")
+        arr = (C.c_void_p * self.fes.dim)(*[f.data_ptr() for f in dfaces])
+        L.check(
+            L.lib().gd_adv_grad_beta_f64(self.handle, adv_coef, u.data_ptr(), r.data_ptr(), dcell.data_ptr(), arr,
+                                         _stream()),
+            "gd_adv_grad_beta_f64",
+        )
+
+
+class _OperatorBase:
+    mass_coef = 0.0
+    adv_coef = 0.0
+
+    def __init__(self, plan: Plan):
+        self.plan = plan
+
+    @property
+    def num_dofs(self):
+        return self.plan.num_dofs
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor):
+        """y = op(x), y overwritten."""
+        return self.plan.apply(self.mass_coef, self.adv_coef, x, y)
+
+    Apply = __call__
+
+    def transpose(self, r: torch.Tensor, z: torch.Tensor):
+        return self.plan.apply_transpose(self.mass_coef, self.adv_coef, r, z)
+
+
+class MassOperator(_OperatorBase):
+    mass_coef = 1.0
+
+
+class AdvectionOperator(_OperatorBase):
+    adv_coef = 1.0
+
+
+class LinearCombinationOperator(_OperatorBase):
+    """mass_coef * M + adv_coef * A on one plan, applied by one fused kernel."""
+
+    def __init__(self, plan: Plan, mass_coef: float, adv_coef: float):
+        super().__init__(plan)
+        self.mass_coef = float(mass_coef)
+        self.adv_coef = float(adv_coef)
+
+
+def MakeMassFiniteElementOperator(fes, ir, sigma=None) -> MassOperator:
+    plan = Plan(fes, ir)
+    plan.set_sigma(sigma)
+    return MassOperator(plan)
+
+
+def MakeAdvectionOperator(fes, ir, adv, bc=None) -> AdvectionOperator:
+    """Without `bc`, boundary faces are skipped (GenDiL's Empty BC); with `bc`
+    the inflow value u+ = g(X) is used on boundary faces."""
+    plan = Plan(fes, ir)
+    plan.set_beta(adv)
+    plan.set_bc(bc)
+    return AdvectionOperator(plan)
+
+
+def MakeMassAdvectionOperator(fes, ir, adv, sigma=None, bc=None, mass_coef=1.0, adv_coef=1.0):
+    """mass_coef * M(sigma) + adv_coef * A(adv, bc) in one pass (GenDiL's fused
+    MassAdvectionOperator, massadvectionoperator.hpp:447, has no boundary terms;
+    here boundary terms follow `bc` as in MakeAdvectionOperator)."""
+    plan = Plan(fes, ir)
+    plan.set_sigma(sigma)
+    plan.set_beta(adv)
+    plan.set_bc(bc)
+    return LinearCombinationOperator(plan, mass_coef, adv_coef)

@@ -1,0 +1,87 @@
+"""Named coefficient fields and synthetic inputs of the benchmark workloads.
+
+  advection_c2(mesh):  beta(x) = (1 + 0.5 sin 2pi y, 0.5 + 0.5 cos 2pi z,
+                       0.25 + 0.25 sin 2pi x) -- each component depends on one
+                       coordinate, so it is a separable field (no HBM traffic).
+  phase_space_6d():    beta = (X3, X4, X5, 0.1, 0, 0) on 3D space x 3D velocity.
+  gaussian_bump(...):  exp(-|x - c|^2 / (2 sigma^2)) at the element nodes.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .discretization import CartesianMesh, FiniteElementSpace, rule_points
+from .fields import AffineField, SeparableField, VectorField
+
+TWO_PI = 2.0 * math.pi
+
+
+def advection_c2() -> VectorField:
+    return VectorField(
+        [
+            SeparableField([(1.0, {1: lambda y: 1.0 + 0.5 * np.sin(TWO_PI * y)})]),
+            SeparableField([(1.0, {2: lambda z: 0.5 + 0.5 * np.cos(TWO_PI * z)})]),
+            SeparableField([(1.0, {0: lambda x: 0.25 + 0.25 * np.sin(TWO_PI * x)})]),
+        ]
+    )
+
+
+def advection_c2_max_norm() -> float:
+    # |beta|^2 = (1+0.5 s1)^2 + (0.5+0.5 c)^2 + (0.25+0.25 s2)^2, maximised at
+    # s1 = 1, c = 1, s2 = 1: 2.25 + 1 + 0.25 = 3.5
+    return math.sqrt(3.5)
+
+
+def phase_space_6d_affine() -> AffineField:
+    A = np.zeros((6, 6))
+    A[0, 3] = A[1, 4] = A[2, 5] = 1.0
+    c = np.zeros(6)
+    c[3] = 0.1
+    return AffineField(A, c)
+
+
+def phase_space_6d_separable() -> VectorField:
+    ident = lambda s: s  # noqa: E731
+    comps = [
+        SeparableField([(1.0, {3: ident})]),
+        SeparableField([(1.0, {4: ident})]),
+        SeparableField([(1.0, {5: ident})]),
+        SeparableField.constant(0.1),
+        SeparableField.constant(0.0),
+        SeparableField.constant(0.0),
+    ]
+    return VectorField(comps)
+
+
+def node_coordinates(fes: FiniteElementSpace) -> np.ndarray:
+    """Physical coordinates of every DoF node, [N, dim] (uniform order)."""
+    m = fes.mesh
+    D = m.dim
+    n = fes.fe.nodes
+    xn, _ = rule_points(fes.fe.basis, n)
+    E = m.num_cells
+    e = np.arange(E)
+    sub = []
+    r = e.copy()
+    for a in range(D):
+        sub.append(r % m.sizes[a])
+        r //= m.sizes[a]
+    loc = np.arange(n ** D)
+    ls = []
+    r = loc.copy()
+    for a in range(D):
+        ls.append(r % n)
+        r //= n
+    X = np.empty((E, n ** D, D))
+    for a in range(D):
+        X[:, :, a] = (m.origin[a] + m.h[a] * sub[a][:, None]) + m.h[a] * xn[ls[a]][None, :]
+    return X.reshape(-1, D)
+
+
+def gaussian_bump(fes: FiniteElementSpace, center, sigma: float) -> np.ndarray:
+    X = node_coordinates(fes)
+    c = np.asarray(center, dtype=np.float64)
+    return np.exp(-np.sum((X - c) ** 2, axis=1) / (2.0 * sigma * sigma))

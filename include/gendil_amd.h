@@ -1,0 +1,228 @@
+/*
+ * gendil_amd.h - C-ABI of the MI355X (gfx950) matrix-free DG operator library.
+ *
+ * This is the drop-in boundary for GenDiL's matrix-free operator apply on
+ * Cartesian hypercube meshes (mass + upwind advection, 1-6D).  Everything
+ * crossing it is a plain pointer, size or POD descriptor; vectors are
+ * device-resident (caller-owned device pointers), applies are stream-ordered
+ * on the caller's hipStream_t (passed as gd_stream, NULL = default stream).
+ *
+ * Every entry point returns an int status (GD_OK = 0); gd_last_error() gives
+ * a thread-local message for the last failure.  Nothing aborts across the ABI
+ * (GenDiL itself aborts via GENDIL_VERIFY, include/gendil/Utilities/debug.hpp:59).
+ *
+ * References below are paths in the GenDiL tree (include/gendil/...).
+ */
+#ifndef GENDIL_AMD_H
+#define GENDIL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GD_MAX_DIM 6
+#define GD_MAX_TERMS 8
+
+typedef struct gd_plan gd_plan; /* opaque operator plan */
+typedef void* gd_stream;        /* hipStream_t */
+
+enum gd_status {
+  GD_OK = 0,
+  GD_ERR_INVALID = 1,     /* bad argument */
+  GD_ERR_UNSUPPORTED = 2, /* valid but not compiled in (dim/order/rule combo) */
+  GD_ERR_HIP = 3,         /* HIP runtime error */
+  GD_ERR_ALLOC = 4,       /* device allocation failed */
+  GD_ERR_NOT_CONVERGED = 5
+};
+
+enum gd_basis { GD_BASIS_GLL = 0, GD_BASIS_GL = 1 };
+enum gd_beta_kind { GD_BETA_SEPARABLE = 0, GD_BETA_POINTWISE = 1, GD_BETA_AFFINE = 2 };
+enum gd_bc_mode {
+  GD_BC_NONE = 0,  /* boundary faces skipped: MakeAdvectionOperator(fes, ir, adv) */
+  GD_BC_INFLOW = 1 /* upwind with u+ = g(X): MakeAdvectionOperator(fes, ir, adv, g) */
+};
+
+/* Scalar field f(X) = sum_r scale[r] * prod_d T_{r,d}(X_d).  Each factor is a
+ * 1D table sampled at the positions gd_plan_axis_points returns for axis d;
+ * offset[r][d] indexes `data`, -1 means the factor is 1.  Replaces GenDiL's
+ * device lambdas sigma(X), adv(X, v), g(X) (massoperator.hpp:270,
+ * advectionoperator.hpp:1416-1528), which cannot cross a C-ABI. */
+typedef struct gd_separable {
+  int32_t rank;
+  double scale[GD_MAX_TERMS];
+  int64_t offset[GD_MAX_TERMS][GD_MAX_DIM];
+  const double* data; /* host pointer, copied at set time */
+  int64_t data_len;
+} gd_separable;
+
+/* Mesh + finite element + integration rule: CartesianMesh<D>(sizes, h, origin,
+ * periodic) (Meshes/MeshDataStructures/CartesianMesh/cartesianndmesh.hpp:16-80),
+ * MakeLobattoFiniteElement / MakeLegendreFiniteElement (finiteelement.hpp:103-109),
+ * MakeIntegrationRule(IntegrationRuleNumPoints<q...>) (integrationrule.hpp:80). */
+typedef struct gd_desc {
+  int32_t dim;
+  int64_t sizes[GD_MAX_DIM];
+  double h[GD_MAX_DIM];
+  double origin[GD_MAX_DIM];
+  int32_t periodic[GD_MAX_DIM];
+  int32_t order;    /* uniform polynomial order p (n = p+1 nodes per dim) */
+  int32_t num_quad; /* uniform Gauss-Legendre points per dim */
+  int32_t basis;    /* gd_basis */
+  /* Optional p-adaptive description (host pointers, length prod(sizes)):
+   * per-element order and rule; NULL = uniform.  No GenDiL counterpart: GenDiL
+   * fixes orders at compile time per space (tensorshapefunctions.hpp:21). */
+  const int8_t* elem_order;
+  const int8_t* elem_quad;
+} gd_desc;
+
+/* ---- plan lifetime ---------------------------------------------------- */
+/* Builds the 1D tables GenDiL builds at operator construction
+ * (bilinearoperator.hpp:20-106, doftoquad.hpp:116-151) and uploads them. */
+int gd_plan_create(const gd_desc* desc, gd_plan** plan);
+int gd_plan_destroy(gd_plan* plan);
+const char* gd_last_error(void);
+int gd_version(void);
+
+int64_t gd_plan_num_dofs(const gd_plan* plan);
+int64_t gd_plan_num_elements(const gd_plan* plan);
+/* Device pointer to the per-element DoF offsets (int64, E+1 entries) of a
+ * p-adaptive plan; NULL for a uniform plan. */
+const int64_t* gd_plan_dof_offsets(const gd_plan* plan);
+/* Sample positions for separable tables on axis d: sizes[d]*(num_quad+2)
+ * values; per element e_d: the num_quad rule points, then xi=0, then xi=1,
+ * each X = (origin + h*e_d) + h*xi (hypercubecell.hpp:41-52). */
+int gd_plan_axis_points(const gd_plan* plan, int32_t axis, double* out);
+/* 1D rule on [0,1]: GL (basis=1) or GLL (basis=0) points and weights. */
+int gd_rule_points(int32_t basis, int32_t npts, double* pts, double* wts);
+
+/* ---- coefficients (set once after create, before applies) -------------- */
+/* mass density sigma(X) (massoperator.hpp:65-83) */
+int gd_plan_set_sigma(gd_plan* plan, const gd_separable* sigma);
+/* advection field beta(X), one separable per component (dim entries) */
+int gd_plan_set_beta_separable(gd_plan* plan, const gd_separable* beta);
+/* beta(X) = c + A X, A row-major dim x dim */
+int gd_plan_set_beta_affine(gd_plan* plan, const double* A, const double* c);
+/* per-point beta (device pointers, caller-owned, read at apply time):
+ * cell [E][dim][q^dim]; face normal component for axis a:
+ * [F_a][q^(dim-1)], F_a = prod_{d!=a} sizes[d] * (sizes[a] + !periodic[a]),
+ * global face f_a between element f_a-1 (+ side) and f_a (- side). */
+int gd_plan_set_beta_pointwise_f64(gd_plan* plan, const double* beta_cell,
+                                   const double* const* beta_face);
+/* inflow boundary: mode gd_bc_mode, g separable (may be NULL for g = 0) */
+int gd_plan_set_bc(gd_plan* plan, int32_t mode, const gd_separable* g);
+
+/* ---- operator apply ---------------------------------------------------- */
+/* y = mass_coef * M(sigma) x + adv_coef * A(beta, g) x, y overwritten (L2
+ * output semantics, massoperator.hpp:209-216, advectionoperator.hpp:1070-1076).
+ * x and y must not alias. */
+int gd_apply_f64(const gd_plan* plan, double mass_coef, double adv_coef,
+                 const double* x, double* y, gd_stream stream);
+int gd_apply_f32(const gd_plan* plan, float mass_coef, float adv_coef,
+                 const float* x, float* y, gd_stream stream);
+/* MassFiniteElementOperator::operator() (massoperator.hpp:209) */
+int gd_mass_apply_f64(const gd_plan* plan, const double* x, double* y, gd_stream stream);
+int gd_mass_apply_f32(const gd_plan* plan, const float* x, float* y, gd_stream stream);
+/* AdvectionOperator::operator() (advectionoperator.hpp:1070) */
+int gd_adv_apply_f64(const gd_plan* plan, const double* u, double* y, gd_stream stream);
+int gd_adv_apply_f32(const gd_plan* plan, const float* u, float* y, gd_stream stream);
+/* Adjoint of the linear part, z = (mass_coef M + adv_coef A)^T r.  No GenDiL
+ * counterpart (GenDiL has no operator adjoint). */
+int gd_apply_T_f64(const gd_plan* plan, double mass_coef, double adv_coef,
+                   const double* r, double* z, gd_stream stream);
+int gd_adv_apply_T_f64(const gd_plan* plan, const double* r, double* z, gd_stream stream);
+/* dL/dbeta for L = <r, adv_coef * A(beta) u> with pointwise beta; writes the
+ * cell and face arrays in the layout of gd_plan_set_beta_pointwise_f64.  No
+ * GenDiL counterpart. */
+int gd_adv_grad_beta_f64(const gd_plan* plan, double adv_coef, const double* u,
+                         const double* r, double* dbeta_cell, double* const* dbeta_face,
+                         gd_stream stream);
+
+/* ---- device memory and vector kernels for the Krylov loops ------------ */
+/* Vector ops used by GMRES_no_alloc / ConjugateGradient (gmres.hpp:74-288,
+ * conjugategradient.hpp:217; vector.hpp:536-628).  GenDiL's Dot is host-only
+ * (Utilities/MathHelperFunctions/dot.hpp:117-130); these stay on the device
+ * and accumulate in FP64. */
+int gd_malloc(void** ptr, size_t bytes);
+int gd_free(void* ptr);
+int gd_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int gd_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int gd_memset(void* dst, int value, size_t bytes, gd_stream stream);
+int gd_stream_sync(gd_stream stream);
+int gd_device_sync(void);
+
+/* result[j] = sum_i V_j[i] * w[i] for j < k; V_j = V + j*ldv (FP64 accumulate),
+ * written to device memory `result` (k doubles). */
+int gd_multidot_f64(int64_t n, int32_t k, const double* V, int64_t ldv, const double* w,
+                    double* result, gd_stream stream);
+int gd_multidot_f32(int64_t n, int32_t k, const float* V, int64_t ldv, const float* w,
+                    double* result, gd_stream stream);
+/* w -= sum_j h[j] * V_j  (h device, FP64) */
+int gd_multiaxpy_f64(int64_t n, int32_t k, const double* h, const double* V, int64_t ldv,
+                     double* w, gd_stream stream);
+int gd_multiaxpy_f32(int64_t n, int32_t k, const double* h, const float* V, int64_t ldv,
+                     float* w, gd_stream stream);
+/* y = a*x + b*y */
+int gd_axpby_f64(int64_t n, double a, const double* x, double b, double* y, gd_stream stream);
+int gd_axpby_f32(int64_t n, float a, const float* x, float b, float* y, gd_stream stream);
+/* precision conversion: y_f32 = scale * x_f64 ; y_f64 = scale * x_f32 */
+int gd_convert_f64_to_f32(int64_t n, double scale, const double* x, float* y, gd_stream stream);
+int gd_convert_f32_to_f64(int64_t n, double scale, const float* x, double* y, gd_stream stream);
+/* y += scale * x (x f32, y f64) */
+int gd_add_f32_to_f64(int64_t n, double scale, const float* x, double* y, gd_stream stream);
+/* One fused classical Gram-Schmidt step with re-orthogonalisation (CGS2)
+ * of w against k <= 32 stored basis vectors V_j (true vector s_j V_j):
+ * hcol[j] = <s_j V_j, w> summed over both passes, w orthogonalised in place,
+ * norm2[0] = ||w||^2 afterwards.  s, c (scratch, k), hcol (k), norm2 (1) are
+ * device arrays.  Replaces the sequential modified Gram-Schmidt of
+ * gmres.hpp:160-165 (m+1 dot/axpy pairs) by three passes over V. */
+int gd_cgs2_step_f64(int64_t n, int32_t k, const double* V, int64_t ldv, double* w, const double* s,
+                     double* c, double* hcol, double* norm2, gd_stream stream);
+int gd_cgs2_step_f32(int64_t n, int32_t k, const float* V, int64_t ldv, float* w, const double* s,
+                     double* c, double* hcol, double* norm2, gd_stream stream);
+/* x += sum_j c[j] V_j (x FP64; V FP64 or FP32; c device, FP64) */
+int gd_basis_update_f64(int64_t n, int32_t k, const double* c, const double* V, int64_t ldv, double* x,
+                        gd_stream stream);
+int gd_basis_update_f32(int64_t n, int32_t k, const double* c, const float* V, int64_t ldv, double* x,
+                        gd_stream stream);
+
+/* ---- Krylov solvers on the operator mass_coef*M + adv_coef*A ------------ */
+typedef struct gd_solve_params {
+  int32_t restart;        /* GMRES restart length m (gmres.hpp:74) */
+  int32_t max_iters;      /* total inner iterations */
+  double rel_tol;         /* on ||b - K x|| / ||b|| */
+  int32_t max_refinements;/* mixed precision: outer FP64 refinement steps */
+  double inner_rel_tol;   /* mixed precision: inner FP32 GMRES tolerance */
+} gd_solve_params;
+
+typedef struct gd_solve_result {
+  int32_t success;
+  int32_t iterations;      /* inner iterations (operator applies in the Arnoldi loop) */
+  int32_t refinements;
+  double final_rel_residual; /* true FP64 residual ||b - K x|| / ||b|| */
+  double seconds;
+} gd_solve_result;
+
+/* Restarted GMRES, all FP64 (GMRES_no_alloc semantics, gmres.hpp:74-288,
+ * with classical Gram-Schmidt applied twice instead of MGS). x holds x0. */
+int gd_gmres_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
+                 double* x, const gd_solve_params* params, gd_solve_result* result,
+                 gd_stream stream);
+/* FP32 operator apply + FP32 Krylov basis inside GMRES; FP64 residual
+ * recompute, FP64 inner products / orthogonalization coefficients and FP64
+ * iterative-refinement update.  No GenDiL counterpart. */
+int gd_gmres_mixed(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
+                   double* x, const gd_solve_params* params, gd_solve_result* result,
+                   gd_stream stream);
+/* Unpreconditioned CG (conjugategradient.hpp:217) for SPD operators (mass). */
+int gd_cg_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
+              double* x, const gd_solve_params* params, gd_solve_result* result,
+              gd_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENDIL_AMD_H */

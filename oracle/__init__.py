@@ -1,0 +1,245 @@
+"""CPU oracle for the MI355X DG operator apply.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+import this package, and only as the checker: the product path never calls it.
+
+  liboracle.so      project-owned CPU restatement of GenDiL's element-centric
+                    apply (gd_oracle.cpp; each function cites the reference).
+  _ref/gendil_ref   GenDiL's own SerialKernelConfiguration operators compiled
+                    from the reference headers (built only where they exist).
+
+Parity status: gdo_apply pinned by golden vectors from _ref/gendil_ref
+(tests/golden); gdo_apply_transpose pinned by dense transposes and the
+dot-product identity; gdo_grad_beta pinned by central finite differences;
+the p-adaptive apply pinned by its uniform special case and by conservation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_BIN = HERE / "_ref" / "gendil_ref"
+
+MAX_DIM = 6
+MAX_TERMS = 8
+
+
+class GdoSeparable(C.Structure):
+    _fields_ = [
+        ("rank", C.c_int32),
+        ("scale", C.c_double * MAX_TERMS),
+        ("offset", (C.c_int64 * MAX_DIM) * MAX_TERMS),
+        ("data", C.POINTER(C.c_double)),
+    ]
+
+
+class GdoProblem(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("sizes", C.c_int64 * MAX_DIM),
+        ("h", C.c_double * MAX_DIM),
+        ("origin", C.c_double * MAX_DIM),
+        ("periodic", C.c_int32 * MAX_DIM),
+        ("order", C.c_int32),
+        ("nq", C.c_int32),
+        ("basis", C.c_int32),
+        ("mass_coef", C.c_double),
+        ("adv_coef", C.c_double),
+        ("sigma", GdoSeparable),
+        ("beta_kind", C.c_int32),
+        ("beta", GdoSeparable * MAX_DIM),
+        ("beta_cell", C.POINTER(C.c_double)),
+        ("beta_face", C.POINTER(C.c_double) * MAX_DIM),
+        ("beta_affine_A", (C.c_double * MAX_DIM) * MAX_DIM),
+        ("beta_affine_c", C.c_double * MAX_DIM),
+        ("bc_mode", C.c_int32),
+        ("bc", GdoSeparable),
+        ("elem_order", C.POINTER(C.c_int8)),
+        ("elem_nq", C.POINTER(C.c_int8)),
+        ("dof_offset", C.POINTER(C.c_int64)),
+        ("num_threads", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        h = C.CDLL(str(LIB_PATH))
+        P = C.POINTER(GdoProblem)
+        pd = C.POINTER(C.c_double)
+        h.gdo_apply.argtypes = [P, pd, pd]
+        h.gdo_apply_transpose.argtypes = [P, pd, pd]
+        h.gdo_grad_beta.argtypes = [P, pd, pd, pd, C.POINTER(pd)]
+        h.gdo_points.argtypes = [C.c_int32, C.c_int32, pd, pd]
+        h.gdo_axis_points.argtypes = [P, C.c_int32, pd]
+        h.gdo_num_dofs.argtypes = [P]
+        h.gdo_num_dofs.restype = C.c_int64
+        _lib = h
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def points(basis: int, n: int):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().gdo_points(basis, n, _dp(x), _dp(w))
+    return x, w
+
+
+class Problem:
+    """Oracle view of a discretization + coefficients (numpy, host)."""
+
+    def __init__(self, fes, ir, mass_coef=0.0, adv_coef=0.0, sigma=None, adv=None, bc=None, num_threads=0):
+        from gendil_amd import fields as F
+        from gendil_amd._lib import BC_INFLOW, BC_NONE
+
+        self.fes = fes
+        self.q = ir.num_points if ir is not None else fes.fe.order + 1
+        m = fes.mesh
+        p = GdoProblem()
+        p.dim = m.dim
+        for d in range(m.dim):
+            p.sizes[d] = m.sizes[d]
+            p.h[d] = m.h[d]
+            p.origin[d] = m.origin[d]
+            p.periodic[d] = int(m.periodic[d])
+        p.order = fes.fe.order
+        p.nq = self.q
+        p.basis = fes.fe.basis
+        p.mass_coef = mass_coef
+        p.adv_coef = adv_coef
+        p.num_threads = num_threads
+        self._keep = []
+        if sigma is None:
+            sigma = F.SeparableField.constant(1.0)
+        elif isinstance(sigma, (int, float)):
+            sigma = F.SeparableField.constant(float(sigma))
+        p.sigma = self._sep(sigma)
+        p.beta_kind = 0
+        if isinstance(adv, F.VectorField):
+            for d, comp in enumerate(adv.components):
+                p.beta[d] = self._sep(comp)
+        elif isinstance(adv, F.AffineField):
+            p.beta_kind = 2
+            A = np.asarray(adv.A, dtype=np.float64).reshape(m.dim, m.dim)
+            c = np.asarray(adv.c, dtype=np.float64).reshape(m.dim)
+            for i in range(m.dim):
+                p.beta_affine_c[i] = c[i]
+                for j in range(m.dim):
+                    p.beta_affine_A[i][j] = A[i, j]
+        elif adv is not None:  # pointwise: numpy (cell, [faces])
+            cell, faces = adv
+            cell = np.ascontiguousarray(cell, dtype=np.float64)
+            faces = [np.ascontiguousarray(f, dtype=np.float64) for f in faces]
+            self._keep += [cell, *faces]
+            p.beta_kind = 1
+            p.beta_cell = _dp(cell)
+            for d, f in enumerate(faces):
+                p.beta_face[d] = _dp(f)
+        if bc is None:
+            p.bc_mode = BC_NONE
+        else:
+            p.bc_mode = BC_INFLOW
+            if isinstance(bc, (int, float)):
+                bc = F.SeparableField.constant(float(bc))
+            p.bc = self._sep(bc)
+        if fes.padaptive:
+            eo = np.ascontiguousarray(fes.elem_order, dtype=np.int8)
+            eq = np.ascontiguousarray(fes.elem_quad, dtype=np.int8)
+            off = np.ascontiguousarray(fes.dof_offsets(), dtype=np.int64)
+            self._keep += [eo, eq, off]
+            p.elem_order = eo.ctypes.data_as(C.POINTER(C.c_int8))
+            p.elem_nq = eq.ctypes.data_as(C.POINTER(C.c_int8))
+            p.dof_offset = off.ctypes.data_as(C.POINTER(C.c_int64))
+        self.p = p
+        self.num_dofs = int(lib().gdo_num_dofs(C.byref(p)))
+
+    def _sep(self, f):
+        s, data = f.tabulate(self.fes.mesh, self.q)
+        self._keep.append(data)
+        o = GdoSeparable()
+        o.rank = s.rank
+        for r in range(MAX_TERMS):
+            o.scale[r] = s.scale[r]
+            for d in range(MAX_DIM):
+                o.offset[r][d] = s.offset[r][d]
+        o.data = _dp(data)
+        return o
+
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.num_dofs)
+        rc = lib().gdo_apply(C.byref(self.p), _dp(x), _dp(y))
+        if rc:
+            raise RuntimeError(f"gdo_apply failed ({rc})")
+        return y
+
+    def apply_transpose(self, r: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.num_dofs)
+        rc = lib().gdo_apply_transpose(C.byref(self.p), _dp(r), _dp(z))
+        if rc:
+            raise RuntimeError(f"gdo_apply_transpose failed ({rc})")
+        return z
+
+    def grad_beta(self, u, r, face_counts):
+        m = self.fes.mesh
+        D = m.dim
+        q = self.q
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        dcell = np.zeros(m.num_cells * D * q ** D)
+        dfaces = [np.zeros(fc * q ** (D - 1)) for fc in face_counts]
+        arr = (C.POINTER(C.c_double) * D)(*[_dp(f) for f in dfaces])
+        rc = lib().gdo_grad_beta(C.byref(self.p), _dp(u), _dp(r), _dp(dcell), arr)
+        if rc:
+            raise RuntimeError(f"gdo_grad_beta failed ({rc})")
+        return dcell, dfaces
+
+
+def ref_available() -> bool:
+    return REF_BIN.exists()
+
+
+def run_ref(args, timeout=600):
+    """Run GenDiL's own apply via the _ref harness (see ref_harness.cpp)."""
+    res = subprocess.run([str(REF_BIN), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    if res.returncode != 0:
+        raise RuntimeError(f"gendil_ref failed: {res.stderr}")
+    return res.stdout
+
+
+def ref_fixture(op, D, P, Q, basis, n, periodic, coef, bc, seed, path):
+    run_ref(["fixture", op, D, P, Q, basis, n, int(periodic), coef, bc, seed, path])
+    d = np.fromfile(path, dtype=np.float64)
+    nd = int(d[:1].view(np.int64)[0])
+    return d[1 : 1 + nd].copy(), d[1 + nd : 1 + 2 * nd].copy()
+
+
+def ref_bench(op, D, P, Q, basis, n, periodic, coef, bc, seed, pairs, env=None, timeout=900):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    res = subprocess.run(
+        [str(REF_BIN), "bench", op, str(D), str(P), str(Q), basis, str(n), str(int(periodic)), coef, bc, str(seed),
+         str(pairs)],
+        capture_output=True, text=True, timeout=timeout, env=e,
+    )
+    if res.returncode != 0:
+        raise RuntimeError(f"gendil_ref bench failed: {res.stderr}")
+    return json.loads(res.stdout.strip().splitlines()[-1])

@@ -1,0 +1,273 @@
+// Harness that instantiates GenDiL's own CPU operators (header-only C++20,
+// SerialKernelConfiguration = OpenMP over elements) and writes input/output
+// vectors for golden fixtures, or times the apply for the CPU baseline.
+//
+// This file is test infrastructure: it is compiled against the reference
+// headers in place (see oracle/Makefile) into oracle/_ref/ and is never
+// linked into the product library.
+//
+// Reference call pattern followed:
+//   tests/integration/gendil/test-mass-3d-generic.cpp:83-150
+//   tests/integration/gendil/test-advection-3d-generic.cpp:84-170
+//   benchmarks/include/benchmark-common.hpp:276-295 (TimeOperator)
+//
+// Usage:
+//   gendil_ref fixture <op> <D> <P> <Q> <basis> <n> <periodic> <coef> <bc> <seed> <out.bin>
+//   gendil_ref bench   <op> <D> <P> <Q> <basis> <n> <periodic> <coef> <bc> <seed> <pairs>
+//   op    : mass | adv
+//   basis : L (Gauss-Lobatto nodes) | G (Gauss-Legendre nodes)
+//   coef  : mass -> one | poly ;  adv -> c2 | bubble | const | trig | phase
+//   bc    : none (boundary faces skipped) | zero | poly
+// fixture output: int64 ndofs, then x[ndofs] (FillRandom(seed)), then y[ndofs].
+
+#include <gendil/gendil.hpp>
+
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <utility>
+
+using namespace gendil;
+
+namespace {
+
+struct Args
+{
+   std::string mode, op, basis, coef, bc;
+   int D = 3, P = 3, Q = 4, n = 4, periodic = 0;
+   unsigned seed = 12345;
+   std::string out;
+   int pairs = 1;
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+template < int D >
+void advection_field( const std::string & kind, const std::array< Real, D > & X, Real ( &v )[D] )
+{
+   if ( kind == "c2" )
+   {
+      if constexpr ( D == 3 )
+      {
+         v[0] = 1.0 + 0.5 * std::sin( 2.0 * kPi * X[1] );
+         v[1] = 0.5 + 0.5 * std::cos( 2.0 * kPi * X[2] );
+         v[2] = 0.25 + 0.25 * std::sin( 2.0 * kPi * X[0] );
+      }
+   }
+   else if ( kind == "bubble" )
+   {
+      for ( int d = 0; d < D; ++d ) v[d] = -X[d] * ( 1.0 - X[d] );
+   }
+   else if ( kind == "const" )
+   {
+      Real c = 1.0;
+      for ( int d = 0; d < D; ++d ) { v[d] = c; c *= -0.5; }
+   }
+   else if ( kind == "trig" )
+   {
+      for ( int d = 0; d < D; ++d ) v[d] = 0.5 + std::sin( 2.0 * kPi * X[( d + 1 ) % D] );
+   }
+   else if ( kind == "phase" )
+   {
+      if constexpr ( D == 6 )
+      {
+         v[0] = X[3];
+         v[1] = X[4];
+         v[2] = X[5];
+         v[3] = 0.1;
+         v[4] = 0.0;
+         v[5] = 0.0;
+      }
+   }
+   else
+   {
+      std::fprintf( stderr, "unknown advection field %s\n", kind.c_str() );
+      std::exit( 2 );
+   }
+}
+
+template < int D >
+Real mass_density( const std::string & kind, const std::array< Real, D > & X )
+{
+   if ( kind == "one" ) return 1.0;
+   if ( kind == "poly" )
+   {
+      if constexpr ( D >= 2 ) return 1.0 + X[0] * X[1] + X[D - 1] * X[D - 1];
+      else return 1.0 + X[0] * X[0];
+   }
+   std::fprintf( stderr, "unknown mass density %s\n", kind.c_str() );
+   std::exit( 2 );
+}
+
+template < int D >
+Real boundary_value( const std::string & kind, const std::array< Real, D > & X )
+{
+   if ( kind == "zero" ) return 0.0;
+   if ( kind == "poly" )
+   {
+      if constexpr ( D >= 2 ) return 1.0 + X[0] - 0.5 * X[1];
+      else return 1.0 + X[0];
+   }
+   std::fprintf( stderr, "unknown boundary field %s\n", kind.c_str() );
+   std::exit( 2 );
+}
+
+template < int D, int P, int Q, bool Lobatto, size_t... Is >
+auto make_space( const Args & a, std::index_sequence< Is... > )
+{
+   std::array< GlobalIndex, D > sizes;
+   std::array< Real, D > h;
+   Point< D > origin;
+   for ( int d = 0; d < D; ++d )
+   {
+      sizes[d] = a.n;
+      h[d] = 1.0 / a.n;
+      origin[d] = 0.0;
+   }
+   CartesianMesh< D > mesh( sizes, h, origin, a.periodic != 0 );
+   FiniteElementOrders< ( (void)Is, P )... > orders;
+   if constexpr ( Lobatto )
+   {
+      auto fe = MakeLobattoFiniteElement( orders );
+      return MakeFiniteElementSpace( mesh, fe );
+   }
+   else
+   {
+      auto fe = MakeLegendreFiniteElement( orders );
+      return MakeFiniteElementSpace( mesh, fe );
+   }
+}
+
+template < int Q, size_t... Is >
+auto make_rule( std::index_sequence< Is... > )
+{
+   IntegrationRuleNumPoints< ( (void)Is, Q )... > nq;
+   return MakeIntegrationRule( nq );
+}
+
+template < typename Op >
+double time_pairs( const Op & op, Vector & x, Vector & y, int pairs )
+{
+   // warm-up pair, then timed pairs (TimeOperator pattern)
+   op( x, y );
+   op( y, x );
+   const auto t0 = std::chrono::steady_clock::now();
+   for ( int i = 0; i < pairs; ++i )
+   {
+      op( x, y );
+      op( y, x );
+   }
+   const auto t1 = std::chrono::steady_clock::now();
+   return std::chrono::duration< double >( t1 - t0 ).count() / ( 2.0 * pairs );
+}
+
+template < typename Op >
+int finish( const Args & a, const Op & op, Integer ndofs, Integer nelem )
+{
+   Vector x( ndofs );
+   Vector y( ndofs );
+   FillRandom( x, a.seed );
+   y = 0.0;
+   if ( a.mode == "fixture" )
+   {
+      op( x, y );
+      const Real * xd = x.ReadHostData();
+      const Real * yd = y.ReadHostData();
+      FILE * f = std::fopen( a.out.c_str(), "wb" );
+      if ( !f ) { std::perror( "fopen" ); return 3; }
+      const long long n = ndofs;
+      std::fwrite( &n, sizeof( n ), 1, f );
+      std::fwrite( xd, sizeof( Real ), ndofs, f );
+      std::fwrite( yd, sizeof( Real ), ndofs, f );
+      std::fclose( f );
+      return 0;
+   }
+   // bench: the two vectors are swapped each apply, as in TimeOperator; scale
+   // the input down so repeated applies stay finite.
+   const double t = time_pairs( op, x, y, a.pairs );
+   std::printf( "{\"seconds_per_apply\": %.9e, \"dofs\": %lld, \"elements\": %lld, \"dofs_per_s\": %.6e}\n",
+                t, (long long)ndofs, (long long)nelem, ndofs / t );
+   return 0;
+}
+
+template < int D, int P, int Q, bool Lobatto >
+int run( const Args & a )
+{
+   auto fes = make_space< D, P, Q, Lobatto >( a, std::make_index_sequence< D >{} );
+   auto ir = make_rule< Q >( std::make_index_sequence< D >{} );
+   const Integer ndofs = fes.GetNumberOfFiniteElementDofs();
+   const Integer nelem = fes.GetNumberOfFiniteElements();
+   const std::string coef = a.coef;
+   const std::string bc = a.bc;
+   if ( a.op == "mass" )
+   {
+      auto sigma = [coef]( const std::array< Real, D > & X ) -> Real { return mass_density< D >( coef, X ); };
+      auto op = MakeMassFiniteElementOperator< SerialKernelConfiguration >( fes, ir, sigma );
+      return finish( a, op, ndofs, nelem );
+   }
+   if ( a.op == "adv" )
+   {
+      auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+      if ( bc == "none" )
+      {
+         auto op = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv );
+         return finish( a, op, ndofs, nelem );
+      }
+      auto g = [bc]( const std::array< Real, D > & X ) -> Real { return boundary_value< D >( bc, X ); };
+      auto op = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv, g );
+      return finish( a, op, ndofs, nelem );
+   }
+   std::fprintf( stderr, "unknown op %s\n", a.op.c_str() );
+   return 2;
+}
+
+using Runner = int ( * )( const Args & );
+struct CaseEntry { int D, P, Q; char basis; Runner fn; };
+
+#define GD_CASE( D, P, Q ) { D, P, Q, 'L', &run< D, P, Q, true > }
+#define GD_CASE_G( D, P, Q ) { D, P, Q, 'G', &run< D, P, Q, false > }
+
+const CaseEntry kEntries[] = {
+   GD_CASE( 3, 0, 2 ), GD_CASE( 3, 1, 2 ), GD_CASE( 3, 1, 3 ), GD_CASE( 3, 2, 3 ),
+   GD_CASE( 3, 2, 4 ), GD_CASE( 3, 2, 5 ), GD_CASE( 3, 3, 4 ), GD_CASE( 3, 3, 5 ),
+   GD_CASE( 3, 3, 6 ), GD_CASE( 3, 4, 5 ), GD_CASE( 3, 4, 6 ),
+   GD_CASE_G( 3, 3, 4 ), GD_CASE_G( 3, 2, 3 ),
+   GD_CASE( 1, 3, 4 ), GD_CASE( 2, 2, 3 ), GD_CASE( 2, 3, 4 ),
+   GD_CASE( 4, 1, 2 ), GD_CASE( 4, 2, 3 ),
+   GD_CASE( 6, 1, 2 ), GD_CASE( 6, 2, 3 ),
+};
+
+} // namespace
+
+int main( int argc, char ** argv )
+{
+   if ( argc < 13 )
+   {
+      std::fprintf( stderr, "usage: %s fixture|bench op D P Q basis n periodic coef bc seed out|pairs\n", argv[0] );
+      return 2;
+   }
+   Args a;
+   a.mode = argv[1];
+   a.op = argv[2];
+   a.D = std::atoi( argv[3] );
+   a.P = std::atoi( argv[4] );
+   a.Q = std::atoi( argv[5] );
+   a.basis = argv[6];
+   a.n = std::atoi( argv[7] );
+   a.periodic = std::atoi( argv[8] );
+   a.coef = argv[9];
+   a.bc = argv[10];
+   a.seed = (unsigned)std::strtoul( argv[11], nullptr, 10 );
+   if ( a.mode == "fixture" ) a.out = argv[12];
+   else a.pairs = std::atoi( argv[12] );
+   for ( const CaseEntry & e : kEntries )
+   {
+      if ( e.D == a.D && e.P == a.P && e.Q == a.Q && e.basis == a.basis[0] ) return e.fn( a );
+   }
+   std::fprintf( stderr, "case D=%d P=%d Q=%d basis=%s not instantiated\n", a.D, a.P, a.Q, a.basis.c_str() );
+   return 2;
+}

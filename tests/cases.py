@@ -1,0 +1,143 @@
+"""Catalogue of parity cases shared by the golden-fixture generator and tests.
+
+Each case mirrors one invocation of oracle/_ref/gendil_ref (GenDiL's own
+operators): same mesh, element, rule and coefficient, described here with the
+fields the C-ABI accepts.  Coefficient names match ref_harness.cpp.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+import gendil_amd as G
+from gendil_amd import workloads as W
+from gendil_amd.fields import SeparableField, VectorField
+
+TWO_PI = 2.0 * math.pi
+
+
+@dataclass(frozen=True)
+class Case:
+    name: str
+    op: str  # mass | adv
+    D: int
+    P: int
+    Q: int
+    basis: str  # L | G
+    n: int
+    periodic: bool
+    coef: str
+    bc: str  # none | zero | poly
+    seed: int = 12345
+
+    def space(self):
+        mesh = G.CartesianMesh.unit(self.D, self.n, self.periodic)
+        fe = G.MakeLobattoFiniteElement(self.P) if self.basis == "L" else G.MakeLegendreFiniteElement(self.P)
+        return G.MakeFiniteElementSpace(mesh, fe), G.MakeIntegrationRule(self.Q)
+
+
+def sigma_field(kind: str, D: int):
+    if kind == "one":
+        return SeparableField.constant(1.0)
+    if kind == "poly":  # 1 + x y + z^2  (1 + x^2 in 1D)
+        if D >= 2:
+            return SeparableField(
+                [(1.0, {}), (1.0, {0: lambda s: s, 1: lambda s: s}), (1.0, {D - 1: lambda s: s * s})]
+            )
+        return SeparableField([(1.0, {}), (1.0, {0: lambda s: s * s})])
+    raise ValueError(kind)
+
+
+def beta_field(kind: str, D: int):
+    if kind == "c2":
+        assert D == 3
+        return W.advection_c2()
+    if kind == "bubble":
+        return VectorField([SeparableField([(1.0, {d: lambda s: -s * (1.0 - s)})]) for d in range(D)])
+    if kind == "const":
+        return VectorField([SeparableField.constant((-0.5) ** d) for d in range(D)])
+    if kind == "trig":
+        return VectorField(
+            [SeparableField([(1.0, {(d + 1) % D: lambda s: 0.5 + np.sin(TWO_PI * s)})]) for d in range(D)]
+        )
+    if kind == "phase":
+        assert D == 6
+        return W.phase_space_6d_separable()
+    raise ValueError(kind)
+
+
+def bc_field(kind: str, D: int):
+    if kind == "none":
+        return None
+    if kind == "zero":
+        return SeparableField.constant(0.0)
+    if kind == "poly":  # 1 + x - 0.5 y  (1 + x in 1D)
+        if D >= 2:
+            return SeparableField([(1.0, {}), (1.0, {0: lambda s: s}), (-0.5, {1: lambda s: s})])
+        return SeparableField([(1.0, {}), (1.0, {0: lambda s: s})])
+    raise ValueError(kind)
+
+
+CASES = [
+    # mass: test-mass-3d-generic.cpp pattern (n=6, GLL p, q=p+3, sigma = 1+xy+z^2)
+    Case("mass3d_p1q3_poly", "mass", 3, 1, 3, "L", 6, False, "poly", "none"),
+    Case("mass3d_p2q5_poly", "mass", 3, 2, 5, "L", 6, False, "poly", "none"),
+    Case("mass3d_p3q6_poly", "mass", 3, 3, 6, "L", 6, False, "poly", "none"),
+    Case("mass3d_p3q4_one", "mass", 3, 3, 4, "L", 4, False, "one", "none"),
+    Case("mass3d_gl_p3q4_poly", "mass", 3, 3, 4, "G", 4, False, "poly", "none"),
+    Case("mass2d_p2q3_poly", "mass", 2, 2, 3, "L", 5, False, "poly", "none"),
+    Case("mass1d_p3q4_poly", "mass", 1, 3, 4, "L", 7, False, "poly", "none"),
+    Case("mass4d_p1q2_poly", "mass", 4, 1, 2, "L", 3, False, "poly", "none"),
+    Case("mass6d_p1q2_one", "mass", 6, 1, 2, "L", 2, False, "one", "none"),
+    # advection: the benchmark field on periodic / inflow / skipped boundaries
+    Case("adv3d_c2_p3q4_per", "adv", 3, 3, 4, "L", 4, True, "c2", "none"),
+    Case("adv3d_c2_p3q4_bc0", "adv", 3, 3, 4, "L", 4, False, "c2", "zero"),
+    Case("adv3d_c2_p3q4_nobc", "adv", 3, 3, 4, "L", 4, False, "c2", "none"),
+    Case("adv3d_c2_p4q5_per", "adv", 3, 4, 5, "L", 3, True, "c2", "none"),
+    Case("adv3d_trig_p3q4_bcpoly", "adv", 3, 3, 4, "L", 3, False, "trig", "poly"),
+    Case("adv3d_trig_gl_p3q4_bcpoly", "adv", 3, 3, 4, "G", 3, False, "trig", "poly"),
+    Case("adv3d_const_p2q4_per", "adv", 3, 2, 4, "L", 4, True, "const", "none"),
+    Case("adv3d_trig_p2q3_m1per", "adv", 3, 2, 3, "L", 1, True, "trig", "none"),
+    # test-advection-3d-generic.cpp pattern (n=2, GLL p=0..3, q=p+2, bubble field, g=0)
+    Case("adv3d_bubble_p0q2", "adv", 3, 0, 2, "L", 2, False, "bubble", "zero"),
+    Case("adv3d_bubble_p1q3", "adv", 3, 1, 3, "L", 2, False, "bubble", "zero"),
+    Case("adv3d_bubble_p2q4", "adv", 3, 2, 4, "L", 2, False, "bubble", "zero"),
+    Case("adv3d_bubble_p3q5", "adv", 3, 3, 5, "L", 2, False, "bubble", "zero"),
+    Case("adv2d_trig_p2q3_bcpoly", "adv", 2, 2, 3, "L", 5, False, "trig", "poly"),
+    Case("adv2d_trig_p3q4_per", "adv", 2, 3, 4, "L", 4, True, "trig", "none"),
+    Case("adv1d_const_p3q4_bcpoly", "adv", 1, 3, 4, "L", 7, False, "const", "poly"),
+    Case("adv4d_trig_p1q2_per", "adv", 4, 1, 2, "L", 3, True, "trig", "none"),
+    Case("adv4d_trig_p2q3_bcpoly", "adv", 4, 2, 3, "L", 3, False, "trig", "poly"),
+    Case("adv6d_phase_p1q2_per", "adv", 6, 1, 2, "L", 2, True, "phase", "none"),
+    Case("adv6d_phase_p2q3_bc0", "adv", 6, 2, 3, "L", 2, False, "phase", "zero"),
+]
+
+BY_NAME = {c.name: c for c in CASES}
+
+
+def oracle_problem(case: Case, **kw):
+    import oracle
+
+    fes, ir = case.space()
+    if case.op == "mass":
+        return oracle.Problem(fes, ir, mass_coef=1.0, sigma=sigma_field(case.coef, case.D), **kw)
+    return oracle.Problem(
+        fes, ir, adv_coef=1.0, adv=beta_field(case.coef, case.D), bc=bc_field(case.bc, case.D), **kw
+    )
+
+
+def gpu_operator(case: Case):
+    fes, ir = case.space()
+    if case.op == "mass":
+        return G.MakeMassFiniteElementOperator(fes, ir, sigma_field(case.coef, case.D))
+    return G.MakeAdvectionOperator(fes, ir, beta_field(case.coef, case.D), bc_field(case.bc, case.D))
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    d = np.linalg.norm(a - b)
+    return float(d / nb) if nb > 0 else float(d)
