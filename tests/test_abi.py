@@ -1,0 +1,68 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every entry
+point include/gendil_amd.h declares, and its host-only helpers behave."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+import gendil_amd as G
+from gendil_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "gendil_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gd_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_are_bound_and_exported():
+    names = declared_symbols()
+    assert len(names) >= 40
+    lib = L.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in L.SIGNATURES, f"{n} missing from the ctypes signature table"
+    assert set(L.SIGNATURES) == set(names)
+
+
+def test_version_and_error_string():
+    lib = L.lib()
+    assert lib.gd_version() == 1
+    rc = lib.gd_rule_points(7, 3, None, None)
+    assert rc == L.GD_ERR_INVALID
+    assert b"gd_rule_points" in lib.gd_last_error()
+
+
+def test_plan_create_rejects_bad_descriptors_without_touching_the_gpu():
+    lib = L.lib()
+    d = L.GdDesc()
+    d.dim = 7
+    h = C.c_void_p()
+    assert lib.gd_plan_create(C.byref(d), C.byref(h)) == L.GD_ERR_INVALID
+    d.dim = 3
+    for i in range(3):
+        d.sizes[i] = 0
+        d.h[i] = 1.0
+    assert lib.gd_plan_create(C.byref(d), C.byref(h)) == L.GD_ERR_INVALID
+    assert not h.value
+
+
+def test_axis_points_layout():
+    mesh = G.CartesianMesh([3], [0.25], [1.0])
+    xs = G.axis_points(mesh, 2, 0).reshape(3, 4)
+    xq, _ = G.rule_points(1, 2)
+    for e in range(3):
+        assert np.allclose(xs[e, :2], 1.0 + 0.25 * e + 0.25 * xq)
+        assert xs[e, 2] == 1.0 + 0.25 * e and xs[e, 3] == (1.0 + 0.25 * e) + 0.25
+
+
+def test_separable_tabulation_offsets():
+    mesh = G.CartesianMesh.unit(3, 4)
+    f = G.SeparableField([(2.0, {1: lambda y: y}), (1.0, {})])
+    s, data = f.tabulate(mesh, 3)
+    assert s.rank == 2 and s.scale[0] == 2.0
+    assert s.offset[0][1] == 0 and s.offset[0][0] == -1 and s.offset[1][1] == -1
+    assert data.size == 4 * 5

@@ -1,0 +1,72 @@
+"""GPU Krylov solves: the FP64 and the FP32-refined GMRES reach the requested
+true FP64 residual, checked independently with the CPU oracle's operator."""
+import numpy as np
+import pytest
+import torch
+
+import gendil_amd as G
+import oracle
+from gendil_amd import solvers as S
+from gendil_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def implicit_step(n):
+    h = 1.0 / n
+    mesh = G.CartesianMesh.unit(3, n, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    ir = G.MakeIntegrationRule(4)
+    dt = 0.5 * h / W.advection_c2_max_norm()
+    K = G.MakeMassAdvectionOperator(fes, ir, W.advection_c2(), bc=0.0, mass_coef=1.0, adv_coef=dt)
+    M = G.MakeMassFiniteElementOperator(fes, ir)
+    u0 = W.gaussian_bump(fes, (0.5, 0.5, 0.5), 0.1) + 1e-3 * G.fill_random(K.num_dofs, 2024)
+    u0t = torch.as_tensor(u0, device="cuda")
+    b = torch.empty_like(u0t)
+    M(u0t, b)
+    Ko = oracle.Problem(fes, ir, mass_coef=1.0, adv_coef=dt, adv=W.advection_c2(), bc=0.0)
+    return K, b, Ko
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed"])
+def test_gmres_reaches_1e10_true_residual(solver):
+    print(f"gmres {solver} on 8^3 p=3", flush=True)
+    K, b, Ko = implicit_step(8)
+    x = torch.zeros_like(b)
+    fn = S.gmres if solver == "fp64" else S.gmres_mixed
+    res = fn(K, b, x, restart=30, rel_tol=1e-10)
+    assert res.success, res
+    xb = x.cpu().numpy()
+    bb = b.cpu().numpy()
+    rel = np.linalg.norm(bb - Ko.apply(xb)) / np.linalg.norm(bb)
+    print(f"{solver}: iters={res.iterations} refinements={res.refinements} oracle residual={rel:.3e}")
+    assert rel <= 2e-10
+
+
+def test_mixed_and_fp64_solutions_agree():
+    K, b, _ = implicit_step(8)
+    x1 = torch.zeros_like(b)
+    x2 = torch.zeros_like(b)
+    S.gmres(K, b, x1, rel_tol=1e-11)
+    S.gmres_mixed(K, b, x2, rel_tol=1e-11)
+    assert float((x1 - x2).norm() / x1.norm()) <= 1e-9
+
+
+def test_cg_on_mass_matrix():
+    mesh = G.CartesianMesh.unit(3, 6, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(2))
+    M = G.MakeMassFiniteElementOperator(fes, G.MakeIntegrationRule(3))
+    xt = torch.as_tensor(G.fill_random(M.num_dofs, 1), device="cuda")
+    b = torch.empty_like(xt)
+    M(xt, b)
+    x = torch.zeros_like(b)
+    res = S.cg(M, b, x, rel_tol=1e-12)
+    assert res.success
+    assert float((x - xt).norm() / xt.norm()) <= 1e-9
+
+
+def test_zero_rhs_gives_zero_solution():
+    K, b, _ = implicit_step(4)
+    x = torch.ones_like(b)
+    res = S.gmres(K, torch.zeros_like(b), x)
+    assert res.success and float(x.abs().max()) == 0.0

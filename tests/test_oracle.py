@@ -1,0 +1,192 @@
+"""CPU tests that pin the oracle (the checker for the HIP kernels).
+
+Each oracle function is pinned by something independent of its own formula:
+GenDiL's own outputs (golden vectors), closed-form quadrature moments, dense
+transposes, finite differences and conservation/textbook special cases.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import gendil_amd as G
+import oracle
+from cases import CASES, beta_field, bc_field, oracle_problem, rel_l2
+from gendil_amd.fields import AffineField, SeparableField, VectorField, face_counts
+
+
+def golden(case):
+    return np.load(f"{oracle.HERE.parent}/tests/golden/{case.name}.npz")
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c.name)
+def test_fill_random_matches_gendil_sequence(case):
+    g = golden(case)
+    x = G.fill_random(int(g["n"]), case.seed)
+    assert hashlib.sha256(x.tobytes()).hexdigest() == str(g["x_sha256"])
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c.name)
+def test_oracle_matches_gendil_golden(case):
+    g = golden(case)
+    x = G.fill_random(int(g["n"]), case.seed)
+    y = oracle_problem(case).apply(x)
+    assert rel_l2(y, g["y"]) <= 1e-13
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_rule_exactness(basis, n):
+    # GL with n points is exact for degree 2n-1, GLL for 2n-3, on [0,1]
+    for pts_fn in (oracle.points, G.rule_points):
+        x, w = pts_fn(basis, n)
+        deg = 2 * n - 1 if basis == 1 else max(0, 2 * n - 3)
+        for k in range(deg + 1):
+            assert abs(np.dot(w, x ** k) - 1.0 / (k + 1)) < 2e-15
+        assert np.allclose(x + x[::-1], 1.0, atol=1e-15)
+        if basis == 0 and n > 1:
+            assert x[0] == 0.0 and x[-1] == 1.0
+
+
+def test_rule_matches_reference_tables():
+    # GaussLegendrePoints<4> and GaussLobattoLegendrePoints<4>, 15 digits
+    # (NumericalIntegration/QuadraturePoints/GLpoints.hpp:131-138, GLLpoints.hpp:132-139)
+    x, w = G.rule_points(1, 4)
+    assert np.allclose(x, [0.0694318442029737, 0.330009478207572, 0.669990521792428, 0.930568155797026], atol=1e-15)
+    assert np.allclose(w, [0.173927422568727, 0.326072577431273, 0.326072577431273, 0.173927422568727], atol=1e-15)
+    x, w = G.rule_points(0, 4)
+    v = 1.0 / np.sqrt(5.0)
+    assert np.allclose(x, [0.0, 0.5 * (1 - v), 0.5 * (1 + v), 1.0], atol=1e-16)
+    assert np.allclose(w, [1 / 12, 5 / 12, 5 / 12, 1 / 12], atol=1e-16)
+    ox, ow = oracle.points(1, 7)
+    gx, gw = G.rule_points(1, 7)
+    assert np.allclose(ox, gx, atol=2e-16) and np.allclose(ow, gw, atol=2e-16)
+
+
+def dense(fn, n):
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = fn(e)
+    return A
+
+
+SMALL = [
+    # (dim, n, p, q, basis, periodic, beta kind, bc)
+    (3, 2, 1, 2, 0, False, "trig", "poly"),
+    (3, 2, 1, 3, 0, True, "trig", "none"),
+    (2, 3, 2, 3, 0, False, "trig", "zero"),
+    (2, 3, 2, 4, 1, True, "const", "none"),
+    (1, 4, 2, 3, 0, False, "const", "poly"),
+]
+
+
+@pytest.mark.parametrize("cfg", SMALL)
+def test_oracle_transpose_is_dense_transpose(cfg):
+    D, n, p, q, basis, per, bk, bck = cfg
+    mesh = G.CartesianMesh.unit(D, n, per)
+    fes = G.FiniteElementSpace(mesh, G.FiniteElement(p, basis))
+    ir = G.MakeIntegrationRule(q)
+    P = oracle.Problem(fes, ir, mass_coef=0.7, adv_coef=1.3, adv=beta_field(bk, D), bc=bc_field(bck, D),
+                       sigma=SeparableField([(1.0, {}), (0.5, {0: lambda s: s})]))
+    N = P.num_dofs
+    # the affine part (inflow g) is not part of the adjoint: use g = 0 for A
+    P0 = oracle.Problem(fes, ir, mass_coef=0.7, adv_coef=1.3, adv=beta_field(bk, D),
+                        bc=None if bck == "none" else 0.0,
+                        sigma=SeparableField([(1.0, {}), (0.5, {0: lambda s: s})]))
+    A = dense(P0.apply, N)
+    AT = dense(P.apply_transpose, N)
+    assert np.abs(AT - A.T).max() <= 1e-13 * max(1.0, np.abs(A).max())
+
+
+def test_oracle_textbook_upwind_1d_p0():
+    # 1D, p=0, constant beta = c > 0, periodic: y_e = c (u_e - u_{e-1})
+    n, c = 7, 0.8
+    mesh = G.CartesianMesh.unit(1, n, True)
+    fes = G.FiniteElementSpace(mesh, G.FiniteElement(0, 0))
+    P = oracle.Problem(fes, G.MakeIntegrationRule(1), adv_coef=1.0, adv=VectorField([SeparableField.constant(c)]))
+    u = G.fill_random(n, 3)
+    y = P.apply(u)
+    assert np.allclose(y, c * (u - np.roll(u, 1)), atol=1e-15)
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c.op == "adv" and c.periodic], ids=lambda c: c.name)
+def test_oracle_conservation_periodic(case):
+    # v = 1 kills the cell term and every face flux enters two elements with
+    # opposite normals: sum_i (A u)_i = 0 (test-advection-p-adaptivity.cpp:106)
+    P = oracle_problem(case)
+    u = G.fill_random(P.num_dofs, 7)
+    y = P.apply(u)
+    assert abs(y.sum()) <= 1e-13 * np.abs(y).sum()
+
+
+def test_oracle_mass_moment():
+    # 1^T M 1 = vol(Omega) for sigma = 1 (Lagrange bases are a partition of unity)
+    for basis in (0, 1):
+        mesh = G.CartesianMesh([3, 2, 4], [0.5, 0.25, 0.1], [0.0, 1.0, -2.0])
+        fes = G.FiniteElementSpace(mesh, G.FiniteElement(2, basis))
+        P = oracle.Problem(fes, G.MakeIntegrationRule(3), mass_coef=1.0)
+        y = P.apply(np.ones(P.num_dofs))
+        assert abs(y.sum() - 1.5 * 0.5 * 0.4) < 1e-14
+
+
+def random_pointwise_beta(mesh, q, rng):
+    D = mesh.dim
+    cell = rng.standard_normal(mesh.num_cells * D * q ** D)
+    faces = [rng.standard_normal(fc * q ** (D - 1)) for fc in face_counts(mesh)]
+    return cell, faces
+
+
+@pytest.mark.parametrize("cfg", [(3, 2, 1, 2, True, None), (2, 3, 2, 3, False, 0.0), (2, 3, 1, 3, False, None)])
+def test_oracle_grad_beta_finite_differences(cfg):
+    D, n, p, q, per, bc = cfg
+    rng = np.random.default_rng(5)
+    mesh = G.CartesianMesh.unit(D, n, per)
+    fes = G.FiniteElementSpace(mesh, G.FiniteElement(p, 0))
+    ir = G.MakeIntegrationRule(q)
+    cell, faces = random_pointwise_beta(mesh, q, rng)
+
+    def make(c, f):
+        return oracle.Problem(fes, ir, adv_coef=1.0, adv=(c, f), bc=bc)
+
+    P = make(cell, faces)
+    u = rng.uniform(-1, 1, P.num_dofs)
+    r = rng.uniform(-1, 1, P.num_dofs)
+    dcell, dfaces = P.grad_beta(u, r, face_counts(mesh))
+    eps = 1e-6
+    for _ in range(8):
+        dc = rng.standard_normal(cell.shape)
+        df = [rng.standard_normal(f.shape) for f in faces]
+        lp = np.dot(r, make(cell + eps * dc, [f + eps * g for f, g in zip(faces, df)]).apply(u))
+        lm = np.dot(r, make(cell - eps * dc, [f - eps * g for f, g in zip(faces, df)]).apply(u))
+        fd = (lp - lm) / (2 * eps)
+        an = np.dot(dcell, dc) + sum(np.dot(a, b) for a, b in zip(dfaces, df))
+        assert abs(fd - an) <= 1e-7 * max(1.0, abs(an))
+
+
+def test_oracle_padaptive_uniform_special_case():
+    # per-element orders that are all equal reproduce the uniform apply
+    case = [c for c in CASES if c.name == "adv3d_c2_p3q4_bc0"][0]
+    fes, ir = case.space()
+    g = np.load(f"{oracle.HERE.parent}/tests/golden/{case.name}.npz")
+    x = G.fill_random(fes.num_dofs, case.seed)
+    E = fes.mesh.num_cells
+    pfes = G.MakePAdaptiveSpace(fes.mesh, np.full(E, 3), np.full(E, 4))
+    P = oracle.Problem(pfes, ir, adv_coef=1.0, adv=beta_field("c2", 3), bc=0.0)
+    assert rel_l2(P.apply(x), g["y"]) <= 1e-13
+
+
+def test_oracle_padaptive_conservation_mixed_orders():
+    rng = np.random.default_rng(11)
+    mesh = G.CartesianMesh.unit(3, 4, True)
+    orders = rng.integers(1, 4, mesh.num_cells)
+    pfes = G.MakePAdaptiveSpace(mesh, orders, orders + 1)
+    A = np.array([[0, 1.0, 0], [0, 0, 1.0], [1.0, 0, 0]])
+    P = oracle.Problem(pfes, None, adv_coef=1.0, adv=AffineField(A, np.array([0.3, -0.2, 0.1])))
+    u = rng.uniform(-1, 1, P.num_dofs)
+    y = P.apply(u)
+    assert abs(y.sum()) <= 1e-13 * np.abs(y).sum()
+    # u = 1 with divergence-free linear beta: A 1 = 0 (exact quadrature)
+    y1 = P.apply(np.ones(P.num_dofs))
+    assert np.abs(y1).max() <= 1e-14
