@@ -1,9 +1,13 @@
 // C-ABI operator applies: argument checks, kernel lookup and launch.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "gd_dispatch.hpp"
+#include "gd_k3d_api.hpp"
 
 namespace gd {
 
@@ -25,6 +29,8 @@ const KernelSet* find_kernels(int D, int N, int Q)
 
 namespace {
 
+int tuned_lanes(const gd_plan* plan, int /*N*/, size_t bytes) { return plan->tuned_k3_lanes[bytes == 8 ? 0 : 1]; }
+
 int lookup(const gd_plan* plan, const KernelSet** ks)
 {
   if (!plan) return fail(GD_ERR_INVALID, "null plan");
@@ -43,6 +49,34 @@ int check_adv(const gd_plan* plan, bool need_pointwise)
   if (need_pointwise && plan->beta_kind != GD_BETA_POINTWISE)
     return fail(GD_ERR_INVALID, "gradient w.r.t. beta needs pointwise beta");
   return GD_OK;
+}
+
+// Lanes per element of the register-resident 3D kernel: from the tuning
+// table (autotuner), overridable with GENDIL_AMD_K3_LANES; 0 = generic kernel.
+template <typename T>
+int k3_lanes(const gd_plan* plan, int N)
+{
+  if (const char* env = std::getenv("GENDIL_AMD_K3_LANES")) return std::atoi(env);
+  const int tuned = tuned_lanes(plan, N, sizeof(T));
+  if (tuned >= 0) return tuned;
+  if (N == 4) return sizeof(T) == 8 ? 2 : 1;
+  return 1;
+}
+
+// The register-resident kernel covers 3D GLL elements with q = p+1 and
+// coefficients that are rank-1 separable (or constant) per component.
+bool k3_eligible(const gd_plan* plan, bool adv, bool mass)
+{
+  const int N = plan->desc.order + 1;
+  if (plan->dim != 3 || plan->padapt || plan->desc.basis != GD_BASIS_GLL || plan->desc.num_quad != N) return false;
+  if (N < 2 || N > 4) return false;
+  if (adv) {
+    if (plan->beta_kind != GD_BETA_SEPARABLE) return false;
+    for (int d = 0; d < 3; ++d)
+      if (plan->beta_host[d].rank > 1) return false;
+  }
+  if (mass && plan->sigma_host.rank > 1) return false;
+  return true;
 }
 
 template <typename T>
@@ -64,6 +98,14 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
   }
   const TableSet* ts = find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
   const Params<T> prm = make_params<T>(plan, ts, mass_coef, adv_coef);
+  if (k3_eligible(plan, adv, mass_coef != T(0))) {
+    const int N = plan->desc.order + 1;
+    const int lanes = k3_lanes<T>(plan, N);
+    if (lanes > 0) {
+      const k3::K3Entry<T>* k = k3::find<T>(N, lanes);
+      if (k) return adv ? k->adv(prm, x, y, s) : k->mass(prm, x, y, s);
+    }
+  }
   if constexpr (std::is_same_v<T, double>) return adv ? ks->fwd64_adv(prm, x, y, s) : ks->fwd64_mass(prm, x, y, s);
   else return adv ? ks->fwd32_adv(prm, x, y, s) : ks->fwd32_mass(prm, x, y, s);
 }
@@ -121,6 +163,78 @@ int gd_apply_T_f64(const gd_plan* plan, double mass_coef, double adv_coef, const
   rc = gd::check_adv(plan, false);
   if (rc) return rc;
   return ks->trn64(prm, r, z, (hipStream_t)stream);
+}
+
+int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, double adv_coef, int32_t reps,
+                     int32_t* chosen, double* best_ms)
+{
+  if (!plan || (precision_bytes != 4 && precision_bytes != 8)) return fail(GD_ERR_INVALID, "autotune: bad arguments");
+  if (reps < 1) reps = 5;
+  const int slot = precision_bytes == 8 ? 0 : 1;
+  const size_t bytes = (size_t)plan->ndofs * precision_bytes;
+  void *x = nullptr, *y = nullptr;
+  if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
+    if (x) (void)hipFree(x);
+    return fail(GD_ERR_ALLOC, "autotune: scratch allocation failed");
+  }
+  (void)hipMemset(x, 0, bytes);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const int saved = plan->tuned_k3_lanes[slot];
+  double best = 1e300;
+  int best_lanes = 0;
+  int rc = GD_OK;
+  for (int lanes : {0, 1, 2, 4}) {
+    plan->tuned_k3_lanes[slot] = lanes;
+    auto run = [&]() -> int {
+      if (precision_bytes == 8)
+        return gd_apply_f64(plan, mass_coef, adv_coef, (const double*)x, (double*)y, nullptr);
+      return gd_apply_f32(plan, (float)mass_coef, (float)adv_coef, (const float*)x, (float*)y, nullptr);
+    };
+    if (lanes > 0) {
+      const int N = plan->desc.order + 1;
+      const bool ok = gd::k3_eligible(plan, adv_coef != 0.0, mass_coef != 0.0) &&
+                      (precision_bytes == 8 ? gd::k3::find<double>(N, lanes) != nullptr
+                                            : gd::k3::find<float>(N, lanes) != nullptr);
+      if (!ok) continue;
+    }
+    rc = run();  // warm-up
+    if (rc) break;
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps && rc == GD_OK; ++r) rc = run();
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    if (rc) break;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double per = ms / reps;
+    if (per < best) {
+      best = per;
+      best_lanes = lanes;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  if (rc) {
+    plan->tuned_k3_lanes[slot] = saved;
+    return rc;
+  }
+  plan->tuned_k3_lanes[slot] = best_lanes;
+  if (chosen) *chosen = best_lanes;
+  if (best_ms) *best_ms = best;
+  return GD_OK;
+}
+
+int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes)
+{
+  if (!plan || (precision_bytes != 4 && precision_bytes != 8) || (lanes != -1 && lanes != 0 && lanes != 1 &&
+                                                                    lanes != 2 && lanes != 4))
+    return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
+  plan->tuned_k3_lanes[precision_bytes == 8 ? 0 : 1] = lanes;
+  return GD_OK;
 }
 
 int gd_adv_apply_T_f64(const gd_plan* plan, const double* r, double* z, gd_stream stream)

@@ -45,6 +45,7 @@ struct Tables1D {
   const T* b1;
   const T* w;
   const T* xq;
+  const T* Gb;  // basis derivative at quadrature points Gb[q][i] = l_i'(x_q)
 };
 
 // Kernel parameters (passed by value as kernel arguments).
@@ -83,7 +84,7 @@ struct TableSet {
   void* dev32 = nullptr;  // contiguous device block (float)
   Tables1D<double> t64{};
   Tables1D<float> t32{};
-  std::vector<double> B, Gq, b0, b1, w, xq;  // host copies
+  std::vector<double> B, Gq, b0, b1, w, xq, Gb;  // host copies
 };
 
 struct HostField {
@@ -122,6 +123,9 @@ struct gd_plan {
   void* fields64 = nullptr;
   void* fields32 = nullptr;
   std::vector<void*> field_tables;  // device table allocations
+  // autotuned lanes per element of the register-resident 3D kernel
+  // ([0] FP64, [1] FP32; -1 = not tuned, 0 = generic kernel)
+  int tuned_k3_lanes[2] = {-1, -1};
   ~gd_plan();
 };
 

@@ -153,7 +153,10 @@ int build_tables(int basis, int n, int q, TableSet& ts)
     ts.b0[i] = lag(xn, i, 0.0);
     ts.b1[i] = lag(xn, i, 1.0);
   }
-  // device upload: B | Gq | b0 | b1 | w | xq
+  ts.Gb.resize((size_t)q * n);
+  for (int a = 0; a < q; ++a)
+    for (int i = 0; i < n; ++i) ts.Gb[a * n + i] = dlag(xn, i, ts.xq[a]);
+  // device upload: B | Gq | b0 | b1 | w | xq | Gb
   std::vector<double> blob;
   blob.insert(blob.end(), ts.B.begin(), ts.B.end());
   blob.insert(blob.end(), ts.Gq.begin(), ts.Gq.end());
@@ -161,6 +164,7 @@ int build_tables(int basis, int n, int q, TableSet& ts)
   blob.insert(blob.end(), ts.b1.begin(), ts.b1.end());
   blob.insert(blob.end(), ts.w.begin(), ts.w.end());
   blob.insert(blob.end(), ts.xq.begin(), ts.xq.end());
+  blob.insert(blob.end(), ts.Gb.begin(), ts.Gb.end());
   std::vector<float> blob32(blob.begin(), blob.end());
   GD_HIP(hipMalloc(&ts.dev64, blob.size() * sizeof(double)));
   GD_HIP(hipMalloc(&ts.dev32, blob32.size() * sizeof(float)));
@@ -179,6 +183,8 @@ int build_tables(int basis, int n, int q, TableSet& ts)
     t.w = base + o;
     o += q;
     t.xq = base + o;
+    o += q;
+    t.Gb = base + o;
   };
   fill(static_cast<double*>(ts.dev64), ts.t64);
   fill(static_cast<float*>(ts.dev32), ts.t32);

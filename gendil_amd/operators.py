@@ -110,6 +110,22 @@ class Plan:
         s, data = bc.tabulate(self.fes.mesh, self.q)
         L.check(L.lib().gd_plan_set_bc(self.handle, L.BC_INFLOW, C.byref(s)), "gd_plan_set_bc")
 
+    # ---- kernel variant selection ----
+    def autotune(self, dtype=torch.float64, mass_coef=0.0, adv_coef=1.0, reps=5):
+        """Time every compiled variant on scratch vectors and keep the fastest.
+        Returns (lanes per element, ms per apply); lanes 0 = generic kernel."""
+        nbytes = 8 if dtype == torch.float64 else 4
+        chosen = C.c_int32()
+        best = C.c_double()
+        L.check(L.lib().gd_plan_autotune(self.handle, nbytes, mass_coef, adv_coef, reps, C.byref(chosen),
+                                         C.byref(best)), "gd_plan_autotune")
+        return chosen.value, best.value
+
+    def set_variant(self, dtype, lanes: int):
+        """Pin the kernel variant: -1 default, 0 generic, 1/2/4 lanes per element."""
+        nbytes = 8 if dtype == torch.float64 else 4
+        L.check(L.lib().gd_plan_set_variant(self.handle, nbytes, lanes), "gd_plan_set_variant")
+
     # ---- applies ----
     def apply(self, mass_coef: float, adv_coef: float, x: torch.Tensor, y: torch.Tensor):
         if x.dtype == torch.float64:

@@ -140,6 +140,18 @@ int gd_adv_grad_beta_f64(const gd_plan* plan, double adv_coef, const double* u,
                          const double* r, double* dbeta_cell, double* const* dbeta_face,
                          gd_stream stream);
 
+/* ---- kernel variant selection (autotuner) ------------------------------ */
+/* Times every compiled variant of the apply for this plan (the generic
+ * LDS-staged kernel, and the register-resident 3D kernel with 1, 2 or 4 lanes
+ * per element where it applies) on scratch vectors, keeps the fastest for
+ * later applies of that precision (4 or 8 bytes), and reports it (0 = generic
+ * kernel).  No GenDiL counterpart: GenDiL fixes its thread layout at compile
+ * time (ThreadBlockLayout, threadlayout.hpp:46). */
+int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, double adv_coef,
+                     int32_t reps, int32_t* chosen, double* best_ms);
+/* Pin a variant: lanes = -1 (default heuristic), 0 (generic), 1, 2 or 4. */
+int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes);
+
 /* ---- device memory and vector kernels for the Krylov loops ------------ */
 /* Vector ops used by GMRES_no_alloc / ConjugateGradient (gmres.hpp:74-288,
  * conjugategradient.hpp:217; vector.hpp:536-628).  GenDiL's Dot is host-only
