@@ -64,19 +64,11 @@ int k3_lanes(const gd_plan* plan, int N)
 }
 
 // The register-resident kernel covers 3D GLL elements with q = p+1 and
-// coefficients that are rank-1 separable (or constant) per component.
-bool k3_eligible(const gd_plan* plan, bool adv, bool mass)
+// one-axis separable (or constant) coefficients; see gd_k3d_host.cpp.
+template <typename T>
+bool k3_eligible(const gd_plan* plan, int lanes, bool adv, bool mass)
 {
-  const int N = plan->desc.order + 1;
-  if (plan->dim != 3 || plan->padapt || plan->desc.basis != GD_BASIS_GLL || plan->desc.num_quad != N) return false;
-  if (N < 2 || N > 4) return false;
-  if (adv) {
-    if (plan->beta_kind != GD_BETA_SEPARABLE) return false;
-    for (int d = 0; d < 3; ++d)
-      if (plan->beta_host[d].rank > 1) return false;
-  }
-  if (mass && plan->sigma_host.rank > 1) return false;
-  return true;
+  return k3::available<T>(plan, lanes, adv, mass);
 }
 
 template <typename T>
@@ -98,12 +90,11 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
   }
   const TableSet* ts = find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
   const Params<T> prm = make_params<T>(plan, ts, mass_coef, adv_coef);
-  if (k3_eligible(plan, adv, mass_coef != T(0))) {
-    const int N = plan->desc.order + 1;
-    const int lanes = k3_lanes<T>(plan, N);
+  {
+    const int lanes = k3_lanes<T>(plan, plan->desc.order + 1);
     if (lanes > 0) {
-      const k3::K3Entry<T>* k = k3::find<T>(N, lanes);
-      if (k) return adv ? k->adv(prm, x, y, s) : k->mass(prm, x, y, s);
+      const int r = k3::apply<T>(plan, lanes, mass_coef, adv_coef, x, y, s);
+      if (r != GD_ERR_UNSUPPORTED) return r;
     }
   }
   if constexpr (std::is_same_v<T, double>) return adv ? ks->fwd64_adv(prm, x, y, s) : ks->fwd64_mass(prm, x, y, s);
@@ -193,10 +184,9 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
       return gd_apply_f32(plan, (float)mass_coef, (float)adv_coef, (const float*)x, (float*)y, nullptr);
     };
     if (lanes > 0) {
-      const int N = plan->desc.order + 1;
-      const bool ok = gd::k3_eligible(plan, adv_coef != 0.0, mass_coef != 0.0) &&
-                      (precision_bytes == 8 ? gd::k3::find<double>(N, lanes) != nullptr
-                                            : gd::k3::find<float>(N, lanes) != nullptr);
+      const bool ok = precision_bytes == 8
+                          ? gd::k3_eligible<double>(plan, lanes, adv_coef != 0.0, mass_coef != 0.0)
+                          : gd::k3_eligible<float>(plan, lanes, adv_coef != 0.0, mass_coef != 0.0);
       if (!ok) continue;
     }
     rc = run();  // warm-up

@@ -122,6 +122,8 @@ struct gd_plan {
   // device copies of separable fields: [0..dim) beta, dim sigma, dim+1 bc
   void* fields64 = nullptr;
   void* fields32 = nullptr;
+  std::vector<gd::FieldDev<double>> fields64_host;  // host copies of the device descriptors
+  std::vector<gd::FieldDev<float>> fields32_host;
   std::vector<void*> field_tables;  // device table allocations
   // autotuned lanes per element of the register-resident 3D kernel
   // ([0] FP64, [1] FP32; -1 = not tuned, 0 = generic kernel)

@@ -4,31 +4,64 @@
 //
 // Mapping: each element is split over LANES adjacent lanes along z; lane h
 // holds the PL = N / LANES z-planes [h*PL, h*PL+PL) of every element tensor
-// in registers.  x- and y-contractions are in-register; z-contractions gather
-// the other lanes' planes with DPP quad broadcasts (no LDS, no barriers).
-// Global loads/stores are element-contiguous (an element's lanes are
-// adjacent, so a wave covers 64/LANES consecutive elements).
+// in registers.  x- and y-contractions are in-register; z-contractions
+// accumulate the other lanes' planes through DPP quad broadcasts (no LDS, no
+// barriers).  Global loads/stores are element-contiguous.
+//
+// The 1D tables are compile-time constants (gd_consttab.hpp), so every
+// contraction is a chain of FMAs against literals.  Coefficient fields are
+// "one-axis" separable: component f(X) = scale_f * T_f(X_{axis_f}) (or a
+// constant, axis -1); the axes are template parameters (AX) so the per-point
+// factor is a register, not a select.
 //
 // What is computed is GenDiL's element-centric operator
 // (massoperator.hpp:39-89, advectionoperator.hpp:106-441), re-factored:
-//   * the cell test-function/gradient contraction uses the basis-derivative
-//     table Gb[q][i] = l_i'(x_q) (equal to GenDiL's collocated Gq * B),
-//     y = Bz^T (Gbx^T By^T D0 + Bx^T Gby^T D1 + Bx^T By^T Dm) + Gbz^T Bx^T By^T D2;
-//   * GLL face traces are slab selections, and the z-face own trace is the
-//     x-y interpolated boundary plane of the cell interpolation;
-//   * when the sign of beta.n is the same at every point of a face across the
-//     whole wave, the trace the upwind flux does not select is skipped.
+//   * the cell gradient/test contraction uses the basis-derivative table
+//     Gb[q][i] = l_i'(x_q) (equal to GenDiL's collocated Gq times B):
+//     y = Bz^T (Gbx^T By^T D0 + Bx^T Gby^T D1 + Bx^T By^T Dm) + Gbz^T Bx^T By^T D2
+//   * GLL face traces are slab selections; the z-face own trace is the
+//     boundary plane of the x-y interpolation of the cell pass;
+//   * when beta.n has one sign at every point of a face across the whole
+//     wave, the trace the upwind flux does not select is not computed.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
+#include "gd_consttab.hpp"
 #include "gd_internal.hpp"
 
 namespace gd {
 namespace k3 {
 
-// ---- cross-lane plane gather ------------------------------------------------
-// Value of v held by lane `src` of this lane's group of LANES lanes.
+template <typename T>
+struct Field1 {
+  T scale;
+  int axis;      // -1: constant; 0..2: table axis
+  const T* tab;  // [M_axis][Q+2] samples (rule points, xi=0, xi=1)
+};
+
+template <typename T>
+struct K3Params {
+  int E, M0, M1, M2;
+  int periodic;  // bit d = periodic axis d
+  int bc_inflow;  // boundary faces: 0 skipped, 1 inflow with u+ = g
+  int num_blocks;
+  T invh[3];
+  T detJ;
+  T adv_coef;
+  T mass_coef;
+  Field1<T> beta[3];
+  Field1<T> sigma;
+  const FieldDev<T>* bc;  // general separable g (BCG instances only)
+};
+
+// axis signature: 4 digits base 4 (axis+1) for beta0, beta1, beta2, sigma
+constexpr int sig(int b0, int b1, int b2, int s) { return (b0 + 1) + 4 * (b1 + 1) + 16 * (b2 + 1) + 64 * (s + 1); }
+constexpr int sig_axis(int S, int f) { return ((S >> (2 * f)) & 3) - 1; }
+
+// ---- cross-lane broadcast within groups of LANES lanes -------------------
 template <int LANES, int SRC>
 __device__ __forceinline__ float bcast(float v)
 {
@@ -54,118 +87,88 @@ __device__ __forceinline__ double bcast(double v)
   }
 }
 
-// full[k][m] for all N planes from own[kk][m] (PL planes per lane)
-template <int LANES, int PL, int M, typename T>
-__device__ __forceinline__ void gather(const T (&own)[PL][M], T (&full)[PL * LANES][M])
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor(F&& f)
 {
-  if constexpr (LANES == 1) {
-#pragma unroll
-    for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-      for (int m = 0; m < M; ++m) full[kk][m] = own[kk][m];
-  } else {
-    [&]<int... S>(std::integer_sequence<int, S...>) {
-      (
-          [&] {
-#pragma unroll
-            for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-              for (int m = 0; m < M; ++m) full[S * PL + kk][m] = bcast<LANES, S>(own[kk][m]);
-          }(),
-          ...);
-    }(std::make_integer_sequence<int, LANES>{});
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    sfor<I + 1, N>(f);
   }
 }
 
-// Cross-lane z-contraction: out[cc][m] = sum_k A(h*PL+cc, k) * in_full[k][m]
-// where A(c, k) = TR ? Mat[k*N + c] : Mat[c*N + k]   (Mat is N x N, uniform)
-template <int N, int LANES, int PL, int M, bool TR, bool ACC, typename T>
-__device__ __forceinline__ void zcontract(const T* __restrict__ Mat, int h, const T (&in)[PL][M], T (&out)[PL][M])
+// Lane-dependent coefficient A[row = h*PL + cc][col] of a constexpr N x N
+// matrix: select among the LANES candidate rows.
+template <int N, int LANES, int PL, typename T, typename Mat>
+__device__ __forceinline__ T lane_row(const Mat& A, int h, int cc, int col)
 {
-  if constexpr (!ACC) {
+  T c = (T)A(cc, col);
 #pragma unroll
-    for (int cc = 0; cc < PL; ++cc)
-#pragma unroll
-      for (int m = 0; m < M; ++m) out[cc][m] = T(0);
-  }
-  // source lanes one at a time: broadcast each of its planes and accumulate,
-  // so no copy of all N planes is ever held
-  [&]<int... S>(std::integer_sequence<int, S...>) {
-    (
-        [&] {
-#pragma unroll
-          for (int kk = 0; kk < PL; ++kk) {
-            const int k = S * PL + kk;
-            T coef[PL];
-#pragma unroll
-            for (int cc = 0; cc < PL; ++cc) {
-              // lane-dependent row h*PL+cc: select among the LANES candidates
-              T c = TR ? Mat[k * N + cc] : Mat[cc * N + k];
-#pragma unroll
-              for (int g = 1; g < LANES; ++g) {
-                const int row = g * PL + cc;
-                const T cg = TR ? Mat[k * N + row] : Mat[row * N + k];
-                c = (h == g) ? cg : c;
-              }
-              coef[cc] = c;
-            }
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-              const T t = bcast<LANES, S>(in[kk][m]);
-#pragma unroll
-              for (int cc = 0; cc < PL; ++cc) out[cc][m] += coef[cc] * t;
-            }
-          }
-        }(),
-        ...);
-  }(std::make_integer_sequence<int, LANES>{});
+  for (int g = 1; g < LANES; ++g) c = (h == g) ? (T)A(g * PL + cc, col) : c;
+  return c;
 }
 
-struct Geo {
-  int e0, e1, e2;   // element sub-indices
-  long long nb[6];  // neighbour per local face (0..2: -x,-y,-z; 3..5: +x,+y,+z), -1 = boundary
+// Plane-at-a-time z contraction: every lane has computed plane P (its local
+// plane cc, global plane c = h*PL + cc);  out[kk][m] += A(k_out, c) * P_c[m]
+// for the lane's output planes k_out = h*PL + kk, summing over the LANES
+// source lanes.  A(k, c) is constexpr.
+template <int N, int LANES, int PL, int M, typename T, typename Mat>
+__device__ __forceinline__ void zacc_plane(const Mat& A, int h, int cc, const T (&P)[M], T (&out)[PL][M])
+{
+  sfor<0, LANES>([&](auto S_) {
+    constexpr int S = decltype(S_)::value;
+    const int c = S * PL + cc;  // global source plane
+    T coef[PL];
+#pragma unroll
+    for (int kk = 0; kk < PL; ++kk) coef[kk] = lane_row<N, LANES, PL, T>([&](int k, int cidx) { return A(k, cidx); }, h, kk, c);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const T t = bcast<LANES, S>(P[m]);
+#pragma unroll
+      for (int kk = 0; kk < PL; ++kk) out[kk][m] += coef[kk] * t;
+    }
+  });
+}
+
+// Full z-interpolation of own planes in[kk][m] (node planes) into out[cc][m]
+// (quadrature planes): out = sum_k A(c_out, k) in_k.
+template <int N, int LANES, int PL, int M, typename T, typename Mat>
+__device__ __forceinline__ void zcontract(const Mat& A, int h, const T (&in)[PL][M], T (&out)[PL][M])
+{
+#pragma unroll
+  for (int cc = 0; cc < PL; ++cc)
+#pragma unroll
+    for (int m = 0; m < M; ++m) out[cc][m] = T(0);
+#pragma unroll
+  for (int kk = 0; kk < PL; ++kk) zacc_plane<N, LANES, PL, M>(A, h, kk, in[kk], out);
+}
+
+template <typename T, int NV>
+struct alignas(16) Chunk {
+  T v[NV];
 };
 
-template <typename T>
-__device__ __forceinline__ Geo geometry(const Params<T>& p, long long e)
+template <typename T, int COUNT>
+__device__ __forceinline__ void copy_block(const T* __restrict__ src, T* __restrict__ dst)
 {
-  Geo g;
-  const long long M0 = p.M[0], M1 = p.M[1], M2 = p.M[2];
-  const long long s1 = M0, s2 = M0 * M1;
-  const long long r = e / M0;
-  g.e0 = (int)(e - r * M0);
-  g.e2 = (int)(r / M1);
-  g.e1 = (int)(r - (long long)g.e2 * M1);
-  const int sub[3] = {g.e0, g.e1, g.e2};
-  const long long Ms[3] = {M0, M1, M2};
-  const long long st[3] = {1, s1, s2};
+  constexpr int VW = 16 / sizeof(T);
+  if constexpr ((COUNT * sizeof(T)) % 16 == 0) {
+    const Chunk<T, VW>* s = reinterpret_cast<const Chunk<T, VW>*>(src);
+    Chunk<T, VW>* d = reinterpret_cast<Chunk<T, VW>*>(dst);
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int a = f % 3;
-    const bool plus = f >= 3;
-    const bool per = (p.periodic_mask >> a) & 1;
-    long long nb;
-    if (Ms[a] == 1) nb = per ? e : -1;
-    else if (!plus) nb = sub[a] > 0 ? e - st[a] : (per ? e + (Ms[a] - 1) * st[a] : -1);
-    else nb = sub[a] < Ms[a] - 1 ? e + st[a] : (per ? e - (Ms[a] - 1) * st[a] : -1);
-    g.nb[f] = nb;
+    for (int v = 0; v < COUNT / VW; ++v) d[v] = s[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < COUNT; ++v) dst[v] = src[v];
   }
-  return g;
 }
 
-// rank-1 separable factor of a field along one axis at sample index k
-template <typename T, int Q>
-__device__ __forceinline__ T factor(const FieldDev<T>& f, int axis, int sub, int k)
+__device__ __forceinline__ int k3_remap(int pb, int nb)
 {
-  const T* tb = f.tab[0][axis];
-  return tb ? tb[sub * (Q + 2) + k] : T(1);
-}
-
-template <typename T, int Q>
-__device__ __forceinline__ T field_at(const FieldDev<T>& f, const int (&sub)[3], int kx, int ky, int kz)
-{
-  if (f.rank == 0) return T(0);
-  return f.scale[0] * factor<T, Q>(f, 0, sub[0], kx) * factor<T, Q>(f, 1, sub[1], ky) * factor<T, Q>(f, 2, sub[2], kz);
+  // blocks are dispatched round-robin over the 8 XCDs: give each XCD one
+  // contiguous range of elements so face neighbours hit its own L2
+  const int per = nb / 8, rem = nb % 8;
+  const int xcd = pb % 8, idx = pb / 8;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
 template <typename T, int Q>
@@ -185,70 +188,217 @@ __device__ __forceinline__ T sep_general(const FieldDev<T>& f, const int (&sub)[
   return sum;
 }
 
-template <typename T, int NV>
-struct alignas(16) Chunk {
-  T v[NV];
+// Per-element samples of a one-axis field along its axis (Q+2 values).
+template <typename T, int Q, int AX>
+struct FieldSamples {
+  T v[AX >= 0 ? Q + 2 : 1];
+  __device__ __forceinline__ void load(const Field1<T>& f, const int (&sub)[3])
+  {
+    if constexpr (AX >= 0) {
+      const T* tb = f.tab + sub[AX] * (Q + 2);
+#pragma unroll
+      for (int k = 0; k < Q + 2; ++k) v[k] = f.scale * tb[k];
+    } else {
+      v[0] = f.scale;
+    }
+  }
+  // value at compile-time sample indices (kx, ky, kz)
+  __device__ __forceinline__ T at(int kx, int ky, int kz) const
+  {
+    if constexpr (AX == 0) return v[kx];
+    else if constexpr (AX == 1) return v[ky];
+    else if constexpr (AX == 2) return v[kz];
+    else return v[0];
+  }
+  // z sample index h*PL + cc, with h lane dependent
+  template <int LANES, int PL>
+  __device__ __forceinline__ T atc(int kx, int ky, int h, int cc) const
+  {
+    if constexpr (AX == 2) {
+      T r = v[cc];
+#pragma unroll
+      for (int g = 1; g < LANES; ++g) r = (h == g) ? v[g * PL + cc] : r;
+      return r;
+    } else {
+      return at(kx, ky, 0);
+    }
+  }
+  // z sample index Q (plus = false) or Q+1, plus lane dependent
+  __device__ __forceinline__ T atzs(int kx, int ky, bool plus) const
+  {
+    if constexpr (AX == 2) return plus ? v[Q + 1] : v[Q];
+    else return at(kx, ky, 0);
+  }
 };
 
-// copy COUNT contiguous values, with 16-byte accesses when the block allows it
-template <typename T, int COUNT>
-__device__ __forceinline__ void copy_block(const T* __restrict__ src, T* __restrict__ dst)
-{
-  constexpr int VW = 16 / sizeof(T);
-  if constexpr ((COUNT * sizeof(T)) % 16 == 0) {
-    const Chunk<T, VW>* s = reinterpret_cast<const Chunk<T, VW>*>(src);
-    Chunk<T, VW>* d = reinterpret_cast<Chunk<T, VW>*>(dst);
-#pragma unroll
-    for (int v = 0; v < COUNT / VW; ++v) d[v] = s[v];
-  } else {
-#pragma unroll
-    for (int v = 0; v < COUNT; ++v) dst[v] = src[v];
-  }
-}
-
-__device__ __forceinline__ int k3_remap(int pb, int nb)
-{
-  const int per = nb / 8, rem = nb % 8;
-  const int xcd = pb % 8, idx = pb / 8;
-  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
-
-template <typename T, int N, int LANES, bool ADV, int NT>
-__global__ void __launch_bounds__(NT) k3d_apply(Params<T> p, const T* __restrict__ x, T* __restrict__ y)
+template <typename T, int N, int LANES, bool ADV, bool MASS, bool BCG, int SIG, int NT>
+__global__ void __launch_bounds__(NT) k3d_apply(K3Params<T> p, const T* __restrict__ x, T* __restrict__ y)
 {
   constexpr int Q = N;
   constexpr int PL = N / LANES;
   constexpr int N2 = N * N, N3 = N * N * N;
   constexpr int EPB = NT / LANES;
+  constexpr auto TB = ct::make_gll_gauss<N, Q>();
+  auto Bm = [](int q, int i) { return TB.B[q][i]; };
+  auto BmT = [](int i, int q) { return TB.B[q][i]; };
+  auto GbT = [](int i, int q) { return TB.Gb[q][i]; };
+
   const int h = LANES == 1 ? 0 : (int)(threadIdx.x % LANES);
   const int blk = k3_remap(blockIdx.x, p.num_blocks);
-  const long long e_raw = (long long)blk * EPB + threadIdx.x / LANES;
+  const int e_raw = blk * EPB + (int)(threadIdx.x / LANES);
   const bool valid = e_raw < p.E;
-  const long long e = valid ? e_raw : p.E - 1;
+  const int e = valid ? e_raw : p.E - 1;
 
-  const T* __restrict__ B = p.tab.B;    // [q][i]
-  const T* __restrict__ Gb = p.tab.Gb;  // basis derivative [q][i]
-  const T* __restrict__ w = p.tab.w;
+  // ---- geometry (32-bit: the host guarantees E < 2^31) ----
+  int sub[3];
+  {
+    const unsigned r = (unsigned)e / (unsigned)p.M0;
+    sub[0] = e - (int)r * p.M0;
+    sub[2] = (int)(r / (unsigned)p.M1);
+    sub[1] = (int)r - sub[2] * p.M1;
+  }
+  auto neighbor = [&](int a, bool plus) -> int {
+    const int Ma = a == 0 ? p.M0 : (a == 1 ? p.M1 : p.M2);
+    const int st = a == 0 ? 1 : (a == 1 ? p.M0 : p.M0 * p.M1);
+    const bool per = (p.periodic >> a) & 1;
+    if (Ma == 1) return per ? e : -1;
+    if (!plus) return sub[a] > 0 ? e - st : (per ? e + (Ma - 1) * st : -1);
+    return sub[a] < Ma - 1 ? e + st : (per ? e - (Ma - 1) * st : -1);
+  };
 
-  const Geo geo = geometry(p, e);
-  const int sub[3] = {geo.e0, geo.e1, geo.e2};
+  constexpr int AX0 = sig_axis(SIG, 0), AX1 = sig_axis(SIG, 1), AX2 = sig_axis(SIG, 2), AXS = sig_axis(SIG, 3);
+  FieldSamples<T, Q, AX0> f0;
+  FieldSamples<T, Q, AX1> f1;
+  FieldSamples<T, Q, AX2> f2;
+  FieldSamples<T, Q, AXS> fs;
+  if constexpr (ADV) {
+    f0.load(p.beta[0], sub);
+    f1.load(p.beta[1], sub);
+    f2.load(p.beta[2], sub);
+  }
+  if constexpr (MASS) fs.load(p.sigma, sub);
 
-  // ---- load own planes: U[kk][j][i] (16-byte vector loads when aligned) ----
+  // ---- own planes ----
   T U[PL][N][N];
-  copy_block<T, PL * N2>(x + e * N3 + (long long)h * PL * N2, &U[0][0][0]);
+  copy_block<T, PL * N2>(x + (long long)e * N3 + h * PL * N2, &U[0][0][0]);
 
-  T Y[PL][N][N];
+  // output accumulator Y[kk][j*N + i]
+  T Y[PL][N * N];
 #pragma unroll
   for (int kk = 0; kk < PL; ++kk)
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-#pragma unroll
-      for (int i = 0; i < N; ++i) Y[kk][j][i] = T(0);
+    for (int m = 0; m < N * N; ++m) Y[kk][m] = T(0);
 
   const T detJ = p.detJ;
-  const bool sep1 = p.beta_kind == GD_BETA_SEPARABLE;  // rank <= 1 checked on the host
+  // Phase boundaries keep the scheduler from hoisting every face's neighbour
+  // loads and tables to the top of the kernel, which would inflate register
+  // pressure far beyond the live data.
+#define GD_K3_PHASE() __builtin_amdgcn_sched_barrier(0)
+  GD_K3_PHASE();
 
-  // ---- x-interpolation: Tx[kk][j][a] ----
+  // boundary value at a face point (sample indices)
+  auto gval = [&](int kx, int ky, int kz) -> T {
+    if constexpr (BCG) return p.bc ? sep_general<T, Q>(*p.bc, sub, kx, ky, kz) : T(0);
+    else return T(0);
+  };
+
+  if constexpr (ADV) {
+    // ================= x faces =================
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      GD_K3_PHASE();
+      const int i_f = s ? N - 1 : 0;
+      const int kn = s ? Q + 1 : Q;
+      const int nb = neighbor(0, s);
+      const bool boundary = nb < 0;
+      const bool active = !(boundary && !p.bc_inflow);
+      const T nrm = (s ? T(1) : T(-1)) * p.invh[0];
+      T dot[PL][Q];
+      bool any_pos = false, any_np = false;
+#pragma unroll
+      for (int cc = 0; cc < PL; ++cc)
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+          const int c = h * PL + cc;
+          dot[cc][b] = f0.template atc<LANES, PL>(kn, b, h, cc) * nrm;
+          any_pos |= dot[cc][b] > T(0);
+          any_np |= !(dot[cc][b] > T(0));
+        }
+      const bool need_own = __any(active && any_pos);
+      const bool need_nb = __any(active && any_np);
+      T own[PL][Q] = {}, nbt[PL][Q] = {};
+      if (need_own) {
+        T t1[PL][Q];
+#pragma unroll
+        for (int kk = 0; kk < PL; ++kk)
+#pragma unroll
+          for (int b = 0; b < Q; ++b) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) acc += (T)TB.B[b][j] * U[kk][j][i_f];
+            t1[kk][b] = acc;
+          }
+        zcontract<N, LANES, PL, Q>(Bm, h, t1, own);
+      }
+      if (need_nb) {
+        const T* src = x + (long long)(boundary ? e : nb) * N3 + h * PL * N2 + (N - 1 - i_f);
+        T t1[PL][Q];
+#pragma unroll
+        for (int kk = 0; kk < PL; ++kk) {
+          T nu[N];
+#pragma unroll
+          for (int j = 0; j < N; ++j) nu[j] = src[kk * N2 + j * N];
+#pragma unroll
+          for (int b = 0; b < Q; ++b) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) acc += (T)TB.B[b][j] * nu[j];
+            t1[kk][b] = acc;
+          }
+        }
+        zcontract<N, LANES, PL, Q>(Bm, h, t1, nbt);
+      }
+      if (need_own || need_nb) {
+        T G[PL][N];
+#pragma unroll
+        for (int kk = 0; kk < PL; ++kk)
+#pragma unroll
+          for (int j = 0; j < N; ++j) G[kk][j] = T(0);
+#pragma unroll
+        for (int cc = 0; cc < PL; ++cc) {
+          const int c = h * PL + cc;
+          T F[Q];
+#pragma unroll
+          for (int b = 0; b < Q; ++b) {
+            const T ext = boundary ? gval(kn, b, c) : nbt[cc][b];
+            const T d = dot[cc][b];
+            const T up = d > T(0) ? d * own[cc][b] : d * ext;
+            F[b] = active ? p.adv_coef * ((T)TB.w[b] * detJ * up) : T(0);
+          }
+          // weight of the c direction is lane dependent: apply per plane
+          T wc = (T)TB.w[cc];
+#pragma unroll
+          for (int g = 1; g < LANES; ++g) wc = (h == g) ? (T)TB.w[g * PL + cc] : wc;
+          T G1[N];
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            T acc = T(0);
+#pragma unroll
+            for (int b = 0; b < Q; ++b) acc += (T)TB.B[b][j] * F[b];
+            G1[j] = wc * acc;
+          }
+          zacc_plane<N, LANES, PL, N>(BmT, h, cc, G1, G);
+        }
+#pragma unroll
+        for (int kk = 0; kk < PL; ++kk)
+#pragma unroll
+          for (int j = 0; j < N; ++j) Y[kk][j * N + i_f] += G[kk][j];
+      }
+    }
+  }
+
+  GD_K3_PHASE();
+  // x-interpolation Tx[kk][j][a]  (U is not needed after this)
   T Tx[PL][N][Q];
 #pragma unroll
   for (int kk = 0; kk < PL; ++kk)
@@ -258,134 +408,34 @@ __global__ void __launch_bounds__(NT) k3d_apply(Params<T> p, const T* __restrict
       for (int a = 0; a < Q; ++a) {
         T s = T(0);
 #pragma unroll
-        for (int i = 0; i < N; ++i) s += B[a * N + i] * U[kk][j][i];
+        for (int i = 0; i < N; ++i) s += (T)TB.B[a][i] * U[kk][j][i];
         Tx[kk][j][a] = s;
       }
 
   if constexpr (ADV) {
-    // ================= x faces (normal along i) =================
+    // ================= y faces =================
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const int f = s ? 3 : 0;
-      const int i_f = s ? N - 1 : 0;
-      const long long nb = geo.nb[f];
-      const bool boundary = nb < 0;
-      const bool active = !(boundary && p.bc_mode == GD_BC_NONE);
-      // dot = beta_x * (+-1/h_x) at points (c = h*PL+cc, b)
-      T dot[PL][Q];
-#pragma unroll
-      for (int cc = 0; cc < PL; ++cc)
-#pragma unroll
-        for (int b = 0; b < Q; ++b) {
-          const int c = h * PL + cc;
-          const T beta = field_at<T, Q>(p.beta[0], sub, s ? Q + 1 : Q, b, c);
-          dot[cc][b] = beta * ((s ? T(1) : T(-1)) * p.invh[0]);
-        }
-      bool any_pos = false, any_nonpos = false;
-#pragma unroll
-      for (int cc = 0; cc < PL; ++cc)
-#pragma unroll
-        for (int b = 0; b < Q; ++b) {
-          any_pos |= dot[cc][b] > T(0);
-          any_nonpos |= !(dot[cc][b] > T(0));
-        }
-      const bool need_own = __any(active && any_pos);
-      const bool need_nb = __any(active && any_nonpos);
-      T own[PL][Q] = {}, nbt[PL][Q] = {};
-      if (need_own) {
-        T t1[PL][Q];
-#pragma unroll
-        for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-          for (int b = 0; b < Q; ++b) {
-            T acc = T(0);
-#pragma unroll
-            for (int j = 0; j < N; ++j) acc += B[b * N + j] * U[kk][j][i_f];
-            t1[kk][b] = acc;
-          }
-        zcontract<N, LANES, PL, Q, false, false>(B, h, t1, own);
-      }
-      if (need_nb) {
-        T nu[PL][N];
-        const long long nbl = boundary ? e : nb;
-        const T* src = x + nbl * N3 + (long long)h * PL * N2 + (N - 1 - i_f);
-#pragma unroll
-        for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-          for (int j = 0; j < N; ++j) nu[kk][j] = src[kk * N2 + j * N];
-        T t1[PL][Q];
-#pragma unroll
-        for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-          for (int b = 0; b < Q; ++b) {
-            T acc = T(0);
-#pragma unroll
-            for (int j = 0; j < N; ++j) acc += B[b * N + j] * nu[kk][j];
-            t1[kk][b] = acc;
-          }
-        zcontract<N, LANES, PL, Q, false, false>(B, h, t1, nbt);
-      }
-      if (need_own || need_nb) {
-        T F[PL][Q];
-#pragma unroll
-        for (int cc = 0; cc < PL; ++cc)
-#pragma unroll
-          for (int b = 0; b < Q; ++b) {
-            const int c = h * PL + cc;
-            T ext;
-            if (boundary) ext = p.bc ? sep_general<T, Q>(*p.bc, sub, s ? Q + 1 : Q, b, c) : T(0);
-            else ext = nbt[cc][b];
-            const T d = dot[cc][b];
-            const T up = d > T(0) ? d * own[cc][b] : d * ext;
-            F[cc][b] = active ? p.adv_coef * ((w[b] * w[c] * detJ) * up) : T(0);
-          }
-        // test functions: y[kk][j][i_f] += sum_c Bz[c][k] sum_b By[b][j] F[c][b]
-        T G1[PL][N];
-#pragma unroll
-        for (int cc = 0; cc < PL; ++cc)
-#pragma unroll
-          for (int j = 0; j < N; ++j) {
-            T acc = T(0);
-#pragma unroll
-            for (int b = 0; b < Q; ++b) acc += B[b * N + j] * F[cc][b];
-            G1[cc][j] = acc;
-          }
-        T G[PL][N];
-        zcontract<N, LANES, PL, N, true, false>(B, h, G1, G);
-#pragma unroll
-        for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-          for (int j = 0; j < N; ++j) Y[kk][j][i_f] += G[kk][j];
-      }
-    }
-
-    // ================= y faces (normal along j) =================
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int f = s ? 4 : 1;
+      GD_K3_PHASE();
       const int j_f = s ? N - 1 : 0;
-      const long long nb = geo.nb[f];
+      const int kn = s ? Q + 1 : Q;
+      const int nb = neighbor(1, s);
       const bool boundary = nb < 0;
-      const bool active = !(boundary && p.bc_mode == GD_BC_NONE);
+      const bool active = !(boundary && !p.bc_inflow);
+      const T nrm = (s ? T(1) : T(-1)) * p.invh[1];
       T dot[PL][Q];
+      bool any_pos = false, any_np = false;
 #pragma unroll
       for (int cc = 0; cc < PL; ++cc)
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
           const int c = h * PL + cc;
-          const T beta = field_at<T, Q>(p.beta[1], sub, a, s ? Q + 1 : Q, c);
-          dot[cc][a] = beta * ((s ? T(1) : T(-1)) * p.invh[1]);
-        }
-      bool any_pos = false, any_nonpos = false;
-#pragma unroll
-      for (int cc = 0; cc < PL; ++cc)
-#pragma unroll
-        for (int a = 0; a < Q; ++a) {
+          dot[cc][a] = f1.template atc<LANES, PL>(a, kn, h, cc) * nrm;
           any_pos |= dot[cc][a] > T(0);
-          any_nonpos |= !(dot[cc][a] > T(0));
+          any_np |= !(dot[cc][a] > T(0));
         }
       const bool need_own = __any(active && any_pos);
-      const bool need_nb = __any(active && any_nonpos);
+      const bool need_nb = __any(active && any_np);
       T own[PL][Q] = {}, nbt[PL][Q] = {};
       if (need_own) {
         T t1[PL][Q];
@@ -393,64 +443,67 @@ __global__ void __launch_bounds__(NT) k3d_apply(Params<T> p, const T* __restrict
         for (int kk = 0; kk < PL; ++kk)
 #pragma unroll
           for (int a = 0; a < Q; ++a) t1[kk][a] = Tx[kk][j_f][a];
-        zcontract<N, LANES, PL, Q, false, false>(B, h, t1, own);
+        zcontract<N, LANES, PL, Q>(Bm, h, t1, own);
       }
       if (need_nb) {
-        T nu[PL][N];
-        const long long nbl = boundary ? e : nb;
-        const T* src = x + nbl * N3 + (long long)h * PL * N2 + (N - 1 - j_f) * N;
-#pragma unroll
-        for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-          for (int i = 0; i < N; ++i) nu[kk][i] = src[kk * N2 + i];
+        const T* src = x + (long long)(boundary ? e : nb) * N3 + h * PL * N2 + (N - 1 - j_f) * N;
         T t1[PL][Q];
 #pragma unroll
-        for (int kk = 0; kk < PL; ++kk)
+        for (int kk = 0; kk < PL; ++kk) {
+          T nu[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) nu[i] = src[kk * N2 + i];
 #pragma unroll
           for (int a = 0; a < Q; ++a) {
             T acc = T(0);
 #pragma unroll
-            for (int i = 0; i < N; ++i) acc += B[a * N + i] * nu[kk][i];
+            for (int i = 0; i < N; ++i) acc += (T)TB.B[a][i] * nu[i];
             t1[kk][a] = acc;
           }
-        zcontract<N, LANES, PL, Q, false, false>(B, h, t1, nbt);
+        }
+        zcontract<N, LANES, PL, Q>(Bm, h, t1, nbt);
       }
       if (need_own || need_nb) {
-        T F[PL][Q];
+        T G[PL][N];
 #pragma unroll
-        for (int cc = 0; cc < PL; ++cc)
+        for (int kk = 0; kk < PL; ++kk)
+#pragma unroll
+          for (int i = 0; i < N; ++i) G[kk][i] = T(0);
+#pragma unroll
+        for (int cc = 0; cc < PL; ++cc) {
+          const int c = h * PL + cc;
+          T wc = (T)TB.w[cc];
+#pragma unroll
+          for (int g = 1; g < LANES; ++g) wc = (h == g) ? (T)TB.w[g * PL + cc] : wc;
+          T F[Q];
 #pragma unroll
           for (int a = 0; a < Q; ++a) {
-            const int c = h * PL + cc;
-            T ext;
-            if (boundary) ext = p.bc ? sep_general<T, Q>(*p.bc, sub, a, s ? Q + 1 : Q, c) : T(0);
-            else ext = nbt[cc][a];
+            const T ext = boundary ? gval(a, kn, c) : nbt[cc][a];
             const T d = dot[cc][a];
             const T up = d > T(0) ? d * own[cc][a] : d * ext;
-            F[cc][a] = active ? p.adv_coef * ((w[a] * w[c] * detJ) * up) : T(0);
+            F[a] = active ? p.adv_coef * ((T)TB.w[a] * detJ * up) : T(0);
           }
-        T G1[PL][N];
-#pragma unroll
-        for (int cc = 0; cc < PL; ++cc)
+          T G1[N];
 #pragma unroll
           for (int i = 0; i < N; ++i) {
             T acc = T(0);
 #pragma unroll
-            for (int a = 0; a < Q; ++a) acc += B[a * N + i] * F[cc][a];
-            G1[cc][i] = acc;
+            for (int a = 0; a < Q; ++a) acc += (T)TB.B[a][i] * F[a];
+            G1[i] = wc * acc;
           }
-        T G[PL][N];
-        zcontract<N, LANES, PL, N, true, false>(B, h, G1, G);
+          zacc_plane<N, LANES, PL, N>(BmT, h, cc, G1, G);
+        }
 #pragma unroll
         for (int kk = 0; kk < PL; ++kk)
 #pragma unroll
-          for (int i = 0; i < N; ++i) Y[kk][j_f][i] += G[kk][i];
+          for (int i = 0; i < N; ++i) Y[kk][j_f * N + i] += G[kk][i];
       }
     }
   }
 
-  // ---- y-interpolation: Txy[kk][b][a] ----
-  T Txy[PL][Q][Q];
+  GD_K3_PHASE();
+  // y-interpolation Txy[kk][b][a]
+  T Txy[PL][Q * Q];
 #pragma unroll
   for (int kk = 0; kk < PL; ++kk)
 #pragma unroll
@@ -459,174 +512,151 @@ __global__ void __launch_bounds__(NT) k3d_apply(Params<T> p, const T* __restrict
       for (int a = 0; a < Q; ++a) {
         T s = T(0);
 #pragma unroll
-        for (int j = 0; j < N; ++j) s += B[b * N + j] * Tx[kk][j][a];
-        Txy[kk][b][a] = s;
+        for (int j = 0; j < N; ++j) s += (T)TB.B[b][j] * Tx[kk][j][a];
+        Txy[kk][b * Q + a] = s;
       }
 
   if constexpr (ADV) {
-    // ================= z faces: the lane holding the boundary plane =================
+    // ================= z faces: lane holding the boundary plane =================
     constexpr int NZ = LANES == 1 ? 2 : 1;
 #pragma unroll
     for (int zz = 0; zz < NZ; ++zz) {
-      // LANES == 1: both faces; otherwise lane 0 takes z-, lane LANES-1 takes z+
+      GD_K3_PHASE();
       const int s = LANES == 1 ? zz : (h == LANES - 1 ? 1 : 0);
       const bool mine = LANES == 1 || h == 0 || h == LANES - 1;
-      const int f = s ? 5 : 2;
       const int kk_f = s ? PL - 1 : 0;
-      const long long nb = geo.nb[f];
+      const int kn = s ? Q + 1 : Q;
+      const int nb = neighbor(2, s);
       const bool boundary = nb < 0;
-      const bool active = mine && !(boundary && p.bc_mode == GD_BC_NONE);
-      T dot[Q][Q];
+      const bool active = mine && !(boundary && !p.bc_inflow);
+      const T nrm = (s ? T(1) : T(-1)) * p.invh[2];
+      T dot[Q * Q];
+      bool any_np = false;
 #pragma unroll
       for (int b = 0; b < Q; ++b)
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
-          const T beta = field_at<T, Q>(p.beta[2], sub, a, b, s ? Q + 1 : Q);
-          dot[b][a] = beta * ((s ? T(1) : T(-1)) * p.invh[2]);
+          dot[b * Q + a] = f2.atzs(a, b, s != 0) * nrm;
+          any_np |= !(dot[b * Q + a] > T(0));
         }
-      bool any_pos = false, any_nonpos = false;
-#pragma unroll
-      for (int b = 0; b < Q; ++b)
-#pragma unroll
-        for (int a = 0; a < Q; ++a) {
-          any_pos |= dot[b][a] > T(0);
-          any_nonpos |= !(dot[b][a] > T(0));
-        }
-      const bool need_nb = __any(active && any_nonpos);
-      T nbt[Q][Q] = {};
+      const bool need_nb = __any(active && any_np);
+      T nbt[Q * Q] = {};
       if (need_nb) {
-        const long long nbl = boundary ? e : nb;
-        const T* src = x + nbl * N3 + (long long)(s ? 0 : N - 1) * N2;
-        T nu[N][N];
-#pragma unroll
-        for (int j = 0; j < N; ++j)
-#pragma unroll
-          for (int i = 0; i < N; ++i) nu[j][i] = src[j * N + i];
+        const T* src = x + (long long)(boundary ? e : nb) * N3 + (s ? 0 : N - 1) * N2;
         T t1[N][Q];
 #pragma unroll
-        for (int j = 0; j < N; ++j)
+        for (int j = 0; j < N; ++j) {
+          T nu[N];
+#pragma unroll
+          for (int i = 0; i < N; ++i) nu[i] = src[j * N + i];
 #pragma unroll
           for (int a = 0; a < Q; ++a) {
             T acc = T(0);
 #pragma unroll
-            for (int i = 0; i < N; ++i) acc += B[a * N + i] * nu[j][i];
+            for (int i = 0; i < N; ++i) acc += (T)TB.B[a][i] * nu[i];
             t1[j][a] = acc;
           }
+        }
 #pragma unroll
         for (int b = 0; b < Q; ++b)
 #pragma unroll
           for (int a = 0; a < Q; ++a) {
             T acc = T(0);
 #pragma unroll
-            for (int j = 0; j < N; ++j) acc += B[b * N + j] * t1[j][a];
-            nbt[b][a] = acc;
+            for (int j = 0; j < N; ++j) acc += (T)TB.B[b][j] * t1[j][a];
+            nbt[b * Q + a] = acc;
           }
       }
       if (__any(active)) {
-        T F[Q][Q];
-#pragma unroll
-        for (int b = 0; b < Q; ++b)
-#pragma unroll
-          for (int a = 0; a < Q; ++a) {
-            T ext;
-            if (boundary) ext = p.bc ? sep_general<T, Q>(*p.bc, sub, a, b, s ? Q + 1 : Q) : T(0);
-            else ext = nbt[b][a];
-            const T d = dot[b][a];
-            const T up = d > T(0) ? d * Txy[kk_f][b][a] : d * ext;
-            F[b][a] = active ? p.adv_coef * ((w[a] * w[b] * detJ) * up) : T(0);
-          }
         T G1[Q][N];
 #pragma unroll
-        for (int b = 0; b < Q; ++b)
+        for (int b = 0; b < Q; ++b) {
+          T F[Q];
+#pragma unroll
+          for (int a = 0; a < Q; ++a) {
+            const T ext = boundary ? gval(a, b, kn) : nbt[b * Q + a];
+            const T d = dot[b * Q + a];
+            // own trace: boundary plane of the x-y interpolation (select, not
+            // a runtime register index, when the plane depends on the lane)
+            const T ownv = (PL == 1 || LANES == 1) ? Txy[kk_f][b * Q + a]
+                                                   : (s ? Txy[PL - 1][b * Q + a] : Txy[0][b * Q + a]);
+            const T up = d > T(0) ? d * ownv : d * ext;
+            F[a] = active ? p.adv_coef * ((T)(TB.w[a] * TB.w[b]) * detJ * up) : T(0);
+          }
 #pragma unroll
           for (int i = 0; i < N; ++i) {
             T acc = T(0);
 #pragma unroll
-            for (int a = 0; a < Q; ++a) acc += B[a * N + i] * F[b][a];
+            for (int a = 0; a < Q; ++a) acc += (T)TB.B[a][i] * F[a];
             G1[b][i] = acc;
           }
+        }
 #pragma unroll
         for (int j = 0; j < N; ++j)
 #pragma unroll
           for (int i = 0; i < N; ++i) {
             T acc = T(0);
 #pragma unroll
-            for (int b = 0; b < Q; ++b) acc += B[b * N + j] * G1[b][i];
-            Y[kk_f][j][i] += acc;
+            for (int b = 0; b < Q; ++b) acc += (T)TB.B[b][j] * G1[b][i];
+            if constexpr (PL == 1 || LANES == 1) {
+              Y[kk_f][j * N + i] += acc;
+            } else {
+              Y[0][j * N + i] += s ? T(0) : acc;
+              Y[PL - 1][j * N + i] += s ? acc : T(0);
+            }
           }
       }
     }
   }
 
-  // ---- z-interpolation: Bu[cc][b][a] for own quadrature planes ----
+  GD_K3_PHASE();
+  // z-interpolation: Bu[cc][b*Q+a] for own quadrature planes
   T Bu[PL][Q * Q];
-  {
-    T in[PL][Q * Q];
-#pragma unroll
-    for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-      for (int b = 0; b < Q; ++b)
-#pragma unroll
-        for (int a = 0; a < Q; ++a) in[kk][b * Q + a] = Txy[kk][b][a];
-    zcontract<N, LANES, PL, Q * Q, false, false>(B, h, in, Bu);
-  }
+  zcontract<N, LANES, PL, Q * Q>(Bm, h, Txy, Bu);
 
-  // quadrature-point coefficient of the cell terms at (a, b, c)
-  auto coef_beta = [&](int d, int a, int b, int c) -> T {
-    const T beta = sep1 ? field_at<T, Q>(p.beta[d], sub, a, b, c) : T(0);
-    return p.adv_coef * (-(w[a] * w[b] * w[c]) * detJ * (beta * p.invh[d]));
-  };
-  auto coef_mass = [&](int a, int b, int c) -> T {
-    return p.mass_coef * ((w[a] * w[b] * w[c]) * detJ * field_at<T, Q>(*p.sigma, sub, a, b, c));
-  };
-
-  // ---- pass A: S2 = Bx^T By^T (c2 Bu);  Y += Gbz^T S2 ----
-  if constexpr (ADV) {
-    T S2[PL][N * N];
+  // ---- cell terms, one quadrature plane at a time, straight into Y ----
+  // c_d(a,b,c) = K_d w_a w_b w_c beta_d,  K_d = -adv_coef detJ / h_d
+  //   S2  = Bx^T By^T (c2 Bu)                          -> Y += Gbz^T S2
+  //   S01 = Gbx^T By^T (c0 Bu) + Bx^T Gby^T (c1 Bu) + Bx^T By^T (m Bu)  -> Y += Bz^T S01
+  const T K0 = -p.adv_coef * detJ * p.invh[0];
+  const T K1 = -p.adv_coef * detJ * p.invh[1];
+  const T K2 = -p.adv_coef * detJ * p.invh[2];
+  const T Km = p.mass_coef * detJ;
 #pragma unroll
-    for (int cc = 0; cc < PL; ++cc) {
-      const int c = h * PL + cc;
-      T E3[N][Q];  // [j][a]
+  for (int cc = 0; cc < PL; ++cc) {
+    GD_K3_PHASE();
+    T wc = (T)TB.w[cc];
+#pragma unroll
+    for (int g = 1; g < LANES; ++g) wc = (h == g) ? (T)TB.w[g * PL + cc] : wc;
+    if constexpr (ADV) {
+      T E3[N][Q];
 #pragma unroll
       for (int a = 0; a < Q; ++a) {
         T v[Q];
 #pragma unroll
-        for (int b = 0; b < Q; ++b) v[b] = coef_beta(2, a, b, c) * Bu[cc][b * Q + a];
+        for (int b = 0; b < Q; ++b)
+          v[b] = (K2 * (T)(TB.w[a] * TB.w[b]) * wc) * f2.template atc<LANES, PL>(a, b, h, cc) * Bu[cc][b * Q + a];
 #pragma unroll
         for (int j = 0; j < N; ++j) {
           T acc = T(0);
 #pragma unroll
-          for (int b = 0; b < Q; ++b) acc += B[b * N + j] * v[b];
+          for (int b = 0; b < Q; ++b) acc += (T)TB.B[b][j] * v[b];
           E3[j][a] = acc;
         }
       }
+      T S2[N * N];
 #pragma unroll
       for (int j = 0; j < N; ++j)
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           T acc = T(0);
 #pragma unroll
-          for (int a = 0; a < Q; ++a) acc += B[a * N + i] * E3[j][a];
-          S2[cc][j * N + i] = acc;
+          for (int a = 0; a < Q; ++a) acc += (T)TB.B[a][i] * E3[j][a];
+          S2[j * N + i] = acc;
         }
+      zacc_plane<N, LANES, PL, N * N>(GbT, h, cc, S2, Y);
     }
-    T Z[PL][N * N];
-    zcontract<N, LANES, PL, N * N, true, false>(Gb, h, S2, Z);
-#pragma unroll
-    for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-      for (int j = 0; j < N; ++j)
-#pragma unroll
-        for (int i = 0; i < N; ++i) Y[kk][j][i] += Z[kk][j * N + i];
-  }
-
-  // ---- pass B: S01 = Gbx^T By^T (c0 Bu) + Bx^T (Gby^T (c1 Bu) + By^T (m Bu));  Y += Bz^T S01 ----
-  {
-    const bool mass = p.mass_coef != T(0);
-    T S01[PL][N * N];
-#pragma unroll
-    for (int cc = 0; cc < PL; ++cc) {
-      const int c = h * PL + cc;
+    {
       T E0[N][Q], E12[N][Q];
 #pragma unroll
       for (int a = 0; a < Q; ++a) {
@@ -634,9 +664,10 @@ __global__ void __launch_bounds__(NT) k3d_apply(Params<T> p, const T* __restrict
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
           const T bu = Bu[cc][b * Q + a];
-          v0[b] = ADV ? coef_beta(0, a, b, c) * bu : T(0);
-          v1[b] = ADV ? coef_beta(1, a, b, c) * bu : T(0);
-          vm[b] = mass ? coef_mass(a, b, c) * bu : T(0);
+          const T wab = (T)(TB.w[a] * TB.w[b]) * wc;
+          v0[b] = ADV ? (K0 * wab) * f0.template atc<LANES, PL>(a, b, h, cc) * bu : T(0);
+          v1[b] = ADV ? (K1 * wab) * f1.template atc<LANES, PL>(a, b, h, cc) * bu : T(0);
+          vm[b] = MASS ? (Km * wab) * fs.template atc<LANES, PL>(a, b, h, cc) * bu : T(0);
         }
 #pragma unroll
         for (int j = 0; j < N; ++j) {
@@ -644,15 +675,16 @@ __global__ void __launch_bounds__(NT) k3d_apply(Params<T> p, const T* __restrict
 #pragma unroll
           for (int b = 0; b < Q; ++b) {
             if constexpr (ADV) {
-              a0 += B[b * N + j] * v0[b];
-              a12 += Gb[b * N + j] * v1[b];
+              a0 += (T)TB.B[b][j] * v0[b];
+              a12 += (T)TB.Gb[b][j] * v1[b];
             }
-            a12 += B[b * N + j] * vm[b];
+            if constexpr (MASS) a12 += (T)TB.B[b][j] * vm[b];
           }
           E0[j][a] = a0;
           E12[j][a] = a12;
         }
       }
+      T S01[N * N];
 #pragma unroll
       for (int j = 0; j < N; ++j)
 #pragma unroll
@@ -660,24 +692,16 @@ __global__ void __launch_bounds__(NT) k3d_apply(Params<T> p, const T* __restrict
           T acc = T(0);
 #pragma unroll
           for (int a = 0; a < Q; ++a) {
-            if constexpr (ADV) acc += Gb[a * N + i] * E0[j][a];
-            acc += B[a * N + i] * E12[j][a];
+            if constexpr (ADV) acc += (T)TB.Gb[a][i] * E0[j][a];
+            acc += (T)TB.B[a][i] * E12[j][a];
           }
-          S01[cc][j * N + i] = acc;
+          S01[j * N + i] = acc;
         }
+      zacc_plane<N, LANES, PL, N * N>(BmT, h, cc, S01, Y);
     }
-    T Z[PL][N * N];
-    zcontract<N, LANES, PL, N * N, true, false>(B, h, S01, Z);
-#pragma unroll
-    for (int kk = 0; kk < PL; ++kk)
-#pragma unroll
-      for (int j = 0; j < N; ++j)
-#pragma unroll
-        for (int i = 0; i < N; ++i) Y[kk][j][i] += Z[kk][j * N + i];
   }
 
-  // ---- store own planes ----
-  if (valid) copy_block<T, PL * N2>(&Y[0][0][0], y + e * N3 + (long long)h * PL * N2);
+  if (valid) copy_block<T, PL * N2>(&Y[0][0], y + (long long)e * N3 + h * PL * N2);
 }
 
 }  // namespace k3

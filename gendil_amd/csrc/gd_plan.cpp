@@ -269,6 +269,8 @@ int upload_fields(gd_plan* p)
       }
     }
   }
+  p->fields64_host = f64;
+  p->fields32_host = f32;
   GD_HIP(hipMalloc(&p->fields64, nf * sizeof(FieldDev<double>)));
   GD_HIP(hipMalloc(&p->fields32, nf * sizeof(FieldDev<float>)));
   GD_HIP(hipMemcpy(p->fields64, f64.data(), nf * sizeof(FieldDev<double>), hipMemcpyHostToDevice));
