@@ -56,6 +56,35 @@ def phase_space_6d_separable() -> VectorField:
     return VectorField(comps)
 
 
+def advection_c2_numpy(X: np.ndarray) -> np.ndarray:
+    """The benchmark field evaluated at points X[..., 3]."""
+    b = np.empty(X.shape[:-1] + (3,))
+    b[..., 0] = 1.0 + 0.5 * np.sin(TWO_PI * X[..., 1])
+    b[..., 1] = 0.5 + 0.5 * np.cos(TWO_PI * X[..., 2])
+    b[..., 2] = 0.25 + 0.25 * np.sin(TWO_PI * X[..., 0])
+    return b
+
+
+def noisy_pointwise_beta(mesh: CartesianMesh, q: int, seed: int = 2024, noise: float = 0.1):
+    """Adjoint workload field: the benchmark field at every cell quadrature
+    point and (normal component) at every face quadrature point, plus
+    noise * N(0,1) drawn from `seed`.  Returns (cell [E, 3, q^3], faces[a]
+    [F_a, q^2]) as float64 numpy arrays."""
+    from .fields import cell_points, face_points
+
+    rng = np.random.default_rng(seed)
+    Xc = cell_points(mesh, q)  # [E, q^3, 3]
+    cell = np.ascontiguousarray(np.transpose(advection_c2_numpy(Xc), (0, 2, 1)))
+    cell += noise * rng.standard_normal(cell.shape)
+    faces = []
+    for a in range(3):
+        Xf = face_points(mesh, q, a)
+        fa = advection_c2_numpy(Xf)[..., a].copy()
+        fa += noise * rng.standard_normal(fa.shape)
+        faces.append(np.ascontiguousarray(fa))
+    return cell, faces
+
+
 def node_coordinates(fes: FiniteElementSpace) -> np.ndarray:
     """Physical coordinates of every DoF node, [N, dim] (uniform order)."""
     m = fes.mesh
