@@ -124,10 +124,18 @@ SIGNATURES = {
     "gd_convert_f64_to_f32": (C.c_int, [_i64, _d, _p, _p, _p]),
     "gd_convert_f32_to_f64": (C.c_int, [_i64, _d, _p, _p, _p]),
     "gd_add_f32_to_f64": (C.c_int, [_i64, _d, _p, _p, _p]),
-    "gd_cgs2_step_f64": (C.c_int, [_i64, _i32, _p, _i64, _p, _p, _p, _p, _p, _p]),
-    "gd_cgs2_step_f32": (C.c_int, [_i64, _i32, _p, _i64, _p, _p, _p, _p, _p, _p]),
-    "gd_basis_update_f64": (C.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),
-    "gd_basis_update_f32": (C.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),
+    "gd_gmres_workspace_bytes": (C.c_size_t, []),
+    "gd_gmres_begin": (C.c_int, [_p, _d, _d, _p]),
+    "gd_arnoldi_step_f64": (C.c_int, [_i64, _i32, _i32, _p, _i64, _p, _p, _p]),
+    "gd_arnoldi_step_f32": (C.c_int, [_i64, _i32, _i32, _p, _i64, _p, _p, _p]),
+    "gd_gmres_update_f64": (C.c_int, [_i64, _i32, _i32, _p, _i64, _p, _p, _p]),
+    "gd_gmres_update_f32": (C.c_int, [_i64, _i32, _i32, _p, _i64, _p, _p, _p]),
+    "gd_host_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "gd_host_free": (C.c_int, [_p]),
+    "gd_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "gd_event_destroy": (C.c_int, [_p]),
+    "gd_event_record": (C.c_int, [_p, _p]),
+    "gd_event_sync": (C.c_int, [_p]),
     "gd_gmres_f64": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),
     "gd_gmres_mixed": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),
     "gd_cg_f64": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),

@@ -1,0 +1,351 @@
+// Device-resident Arnoldi process for restarted GMRES(m), m <= 32.
+//
+// One iteration k (basis vectors V_0..V_k stored with scales s_j, true vector
+// v_j = s_j V_j, and W = V_{k+1} holding K V_k) is three fused passes over the
+// basis, each ending in a deterministic reduction done by the last block to
+// finish (block partials summed in block order):
+//   pass A: h1_j = s_j <V_j, W>                         (classical GS)
+//   pass B: W -= sum_j s_j h1_j V_j, then h2_j = s_j <V_j, W>, ||W||^2
+//           (re-orthogonalisation, the "twice is enough" CGS2 step)
+//   pass C: Z = (W - sum_j s_j h2_j V_j) / p with p^2 = ||W||^2 - sum_j h2_j^2
+//           (Pythagoras), stored as V_{k+1}; nu = ||Z|| exactly; s_{k+1} = 1/nu
+// and the last block of pass C finishes the Hessenberg column
+// H_{jk} = s_k (h1_j + h2_j), H_{k+1,k} = s_k p nu, applies the previous Givens
+// rotations and forms the new one (gmres.hpp:170-210), and writes the residual
+// estimate |g_{k+1}| to an optional caller pointer (pinned host memory, so the
+// host can read it after an event with no copy in the stream).  Stored vectors
+// are kept at unit norm (up to rounding), so the operator is applied with
+// scale 1 and FP32 bases cannot drift out of range.
+//
+// The cycle end solves the triangular system H y = g in every block (k <= 32,
+// cheap) and adds x += sum_j s_j y_j V_j in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "gd_internal.hpp"
+
+namespace gd {
+namespace arnoldi {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxM = 32;          // restart length
+constexpr int kMaxBlocks = 1024;
+constexpr int kStride = kMaxM + 2; // partial columns per block: k+1 dots + norm
+
+struct Workspace {
+  double H[kMaxM][kMaxM + 1];  // column k, rows 0..k+1 (rotated in place)
+  double cs[kMaxM], sn[kMaxM];
+  double g[kMaxM + 1];
+  double s[kMaxM + 1];
+  double c[kMaxM];   // coefficients on the stored vectors for the next update pass
+  double h1[kMaxM];  // accumulated true GS coefficients of the current column
+  double pnorm, pinv;
+  unsigned int counter[4];
+  double partial[kMaxBlocks][kStride];
+};
+
+inline int grid_for(int64_t n)
+{
+  const int64_t b = (n + kThreads * 4 - 1) / (kThreads * 4);
+  return (int)(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block sums of the first `cols` accumulators into partial[blockIdx.x][:].
+template <int K>
+__device__ __forceinline__ void store_block_partials(const double (&acc)[K], int cols, double* out)
+{
+  __shared__ double red[kWaves][K];
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (j < cols) {
+      const double v = wave_sum(acc[j]);
+      if (lane == 0) red[wv][j] = v;
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < cols; j += kThreads) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s += red[w][j];
+    out[j] = s;
+  }
+}
+
+// True in the one block that finishes last; its reads of the other blocks'
+// partials are ordered after their writes (release/acquire at device scope).
+// The last block resets the counter for the next launch.
+__device__ __forceinline__ bool last_block(unsigned int* counter)
+{
+  __shared__ bool is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    if (threadIdx.x == 0) *counter = 0u;
+  }
+  return is_last;
+}
+
+// out[j] = sum over blocks of partial[b][j], j < cols: one wave per column,
+// fixed order.  Results land in shared memory `col` (all threads see them
+// after the trailing barrier).
+__device__ __forceinline__ void sum_columns(const Workspace* ws, int nblk, int cols, double* col)
+{
+  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+  for (int j = wv; j < cols; j += kWaves) {
+    double v = 0.0;
+    for (int b = lane; b < nblk; b += 64) v += ws->partial[b][j];
+    v = wave_sum(v);
+    if (lane == 0) col[j] = v;
+  }
+  __syncthreads();
+}
+
+// Pass A.
+template <typename TV>
+__global__ void __launch_bounds__(kThreads) k_pass_a(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
+                                                     Workspace* __restrict__ ws)
+{
+  const int kk = k + 1;
+  const TV* __restrict__ w = V + (int64_t)(k + 1) * ldv;
+  double acc[kMaxM];
+#pragma unroll
+  for (int j = 0; j < kMaxM; ++j) acc[j] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const double wi = (double)w[i];
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j)
+      if (j < kk) acc[j] += (double)V[j * ldv + i] * wi;
+  }
+  store_block_partials<kMaxM>(acc, kk, ws->partial[blockIdx.x]);
+  if (!last_block(&ws->counter[0])) return;
+  __shared__ double col[kStride];
+  sum_columns(ws, gridDim.x, kk, col);
+  for (int j = threadIdx.x; j < kk; j += kThreads) {
+    const double h = ws->s[j] * col[j];
+    ws->h1[j] = h;
+    ws->c[j] = ws->s[j] * h;
+  }
+}
+
+// Pass B.
+template <typename TV>
+__global__ void __launch_bounds__(kThreads) k_pass_b(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
+                                                     Workspace* __restrict__ ws)
+{
+  const int kk = k + 1;
+  TV* __restrict__ w = const_cast<TV*>(V) + (int64_t)(k + 1) * ldv;
+  double cj[kMaxM];
+#pragma unroll
+  for (int j = 0; j < kMaxM; ++j) cj[j] = j < kk ? ws->c[j] : 0.0;
+  double acc[kMaxM + 1];
+#pragma unroll
+  for (int j = 0; j <= kMaxM; ++j) acc[j] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    double v[kMaxM];
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j) {
+      v[j] = j < kk ? (double)V[j * ldv + i] : 0.0;
+      s += cj[j] * v[j];
+    }
+    const TV wt = (TV)((double)w[i] - s);
+    w[i] = wt;
+    const double wr = (double)wt;
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j) acc[j] += v[j] * wr;
+    acc[kMaxM] += wr * wr;
+  }
+  // move the norm accumulator to column kk so the partial row is dense
+#pragma unroll
+  for (int j = 0; j < kMaxM; ++j)
+    if (j == kk) acc[j] = acc[kMaxM];
+  store_block_partials<kMaxM + 1>(acc, kk + 1, ws->partial[blockIdx.x]);
+  if (!last_block(&ws->counter[1])) return;
+  __shared__ double col[kStride];
+  sum_columns(ws, gridDim.x, kk + 1, col);
+  if (threadIdx.x == 0) {
+    double ss = 0.0;
+    for (int j = 0; j < kk; ++j) {
+      const double h2 = ws->s[j] * col[j];
+      ws->h1[j] += h2;
+      ws->c[j] = ws->s[j] * h2;
+      ss += h2 * h2;
+    }
+    const double p2 = col[kk] - ss;
+    const double p = p2 > 0.0 ? sqrt(p2) : 0.0;
+    ws->pnorm = p;
+    ws->pinv = p > 0.0 ? 1.0 / p : 0.0;
+  }
+}
+
+// Pass C, with the Hessenberg column and its Givens rotation.
+template <typename TV>
+__global__ void __launch_bounds__(kThreads) k_pass_c(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
+                                                     Workspace* __restrict__ ws, double* __restrict__ resid_out)
+{
+  const int kk = k + 1;
+  TV* __restrict__ w = const_cast<TV*>(V) + (int64_t)(k + 1) * ldv;
+  const double pinv = ws->pinv;
+  double cj[kMaxM];
+#pragma unroll
+  for (int j = 0; j < kMaxM; ++j) cj[j] = j < kk ? ws->c[j] : 0.0;
+  double acc[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j)
+      if (j < kk) s += cj[j] * (double)V[j * ldv + i];
+    const TV zt = (TV)(((double)w[i] - s) * pinv);
+    w[i] = zt;
+    const double zr = (double)zt;
+    acc[0] += zr * zr;
+  }
+  store_block_partials<1>(acc, 1, ws->partial[blockIdx.x]);
+  if (!last_block(&ws->counter[2])) return;
+  __shared__ double col[kStride];
+  sum_columns(ws, gridDim.x, 1, col);
+  if (threadIdx.x == 0) {
+    const double nu = sqrt(col[0]);
+    const double sk = ws->s[k];
+    ws->s[k + 1] = nu > 0.0 ? 1.0 / nu : 0.0;
+    double* h = ws->H[k];
+    for (int j = 0; j < kk; ++j) h[j] = sk * ws->h1[j];
+    h[k + 1] = sk * ws->pnorm * nu;
+    for (int i = 0; i < k; ++i) {
+      const double a = h[i], b = h[i + 1];
+      h[i] = ws->cs[i] * a + ws->sn[i] * b;
+      h[i + 1] = -ws->sn[i] * a + ws->cs[i] * b;
+    }
+    const double a = h[k], b = h[k + 1];
+    const double rho = hypot(a, b);
+    const double cs = rho > 0.0 ? a / rho : 1.0, sn = rho > 0.0 ? b / rho : 0.0;
+    ws->cs[k] = cs;
+    ws->sn[k] = sn;
+    h[k] = rho;
+    h[k + 1] = 0.0;
+    const double gk = ws->g[k];
+    ws->g[k + 1] = -sn * gk;
+    ws->g[k] = cs * gk;
+    if (resid_out) *resid_out = fabs(ws->g[k + 1]);
+  }
+}
+
+// Cycle start: g = beta e_0, s_0 = s0, counters cleared.
+__global__ void k_begin(double beta, double s0, Workspace* __restrict__ ws)
+{
+  const int t = threadIdx.x;
+  if (t <= kMaxM) ws->g[t] = t == 0 ? beta : 0.0;
+  if (t == 0) ws->s[0] = s0;
+  if (t < 4) ws->counter[t] = 0u;
+}
+
+// Cycle end: y = H(0:k,0:k)^{-1} g(0:k) in every block, x += sum_j s_j y_j V_j.
+template <typename TV>
+__global__ void __launch_bounds__(kThreads) k_update_x(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
+                                                       const Workspace* __restrict__ ws, double* __restrict__ x)
+{
+  __shared__ double coef[kMaxM];
+  if (threadIdx.x == 0) {
+    double y[kMaxM];
+    for (int i = k - 1; i >= 0; --i) {
+      double acc = ws->g[i];
+      for (int j = i + 1; j < k; ++j) acc -= ws->H[j][i] * y[j];
+      y[i] = acc / ws->H[i][i];
+    }
+    for (int j = 0; j < k; ++j) coef[j] = ws->s[j] * y[j];
+  }
+  __syncthreads();
+  double cj[kMaxM];
+#pragma unroll
+  for (int j = 0; j < kMaxM; ++j) cj[j] = j < k ? coef[j] : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxM; ++j)
+      if (j < k) s += cj[j] * (double)V[j * ldv + i];
+    x[i] += s;
+  }
+}
+
+template <typename TV>
+int step(int64_t n, int m, int k, TV* V, int64_t ldv, void* wsp, double* resid_out, hipStream_t st)
+{
+  if (n < 1 || m < 1 || m > kMaxM || k < 0 || k >= m || !V || !wsp || ldv < n)
+    return fail(GD_ERR_INVALID, "gd_arnoldi_step: bad arguments");
+  Workspace* ws = static_cast<Workspace*>(wsp);
+  const int nb = grid_for(n);
+  hipLaunchKernelGGL((k_pass_a<TV>), dim3(nb), dim3(kThreads), 0, st, n, k, V, ldv, ws);
+  hipLaunchKernelGGL((k_pass_b<TV>), dim3(nb), dim3(kThreads), 0, st, n, k, V, ldv, ws);
+  hipLaunchKernelGGL((k_pass_c<TV>), dim3(nb), dim3(kThreads), 0, st, n, k, V, ldv, ws, resid_out);
+  GD_HIP(hipGetLastError());
+  return GD_OK;
+}
+
+template <typename TV>
+int update(int64_t n, int m, int k, const TV* V, int64_t ldv, const void* wsp, double* x, hipStream_t st)
+{
+  if (n < 1 || m < 1 || m > kMaxM || k < 0 || k > m || !V || !wsp || !x || ldv < n)
+    return fail(GD_ERR_INVALID, "gd_gmres_update: bad arguments");
+  if (k == 0) return GD_OK;
+  hipLaunchKernelGGL((k_update_x<TV>), dim3(grid_for(n)), dim3(kThreads), 0, st, n, k, V, ldv,
+                     static_cast<const Workspace*>(wsp), x);
+  GD_HIP(hipGetLastError());
+  return GD_OK;
+}
+
+}  // namespace arnoldi
+}  // namespace gd
+
+using gd::fail;
+
+extern "C" {
+
+size_t gd_gmres_workspace_bytes(void) { return sizeof(gd::arnoldi::Workspace); }
+
+int gd_gmres_begin(void* ws, double beta, double s0, gd_stream stream)
+{
+  if (!ws) return fail(GD_ERR_INVALID, "gd_gmres_begin: null workspace");
+  hipLaunchKernelGGL(gd::arnoldi::k_begin, dim3(1), dim3(64), 0, (hipStream_t)stream, beta, s0,
+                     static_cast<gd::arnoldi::Workspace*>(ws));
+  GD_HIP(hipGetLastError());
+  return GD_OK;
+}
+
+int gd_arnoldi_step_f64(int64_t n, int32_t m, int32_t k, double* V, int64_t ldv, void* ws, double* resid_out,
+                        gd_stream stream)
+{
+  return gd::arnoldi::step<double>(n, m, k, V, ldv, ws, resid_out, (hipStream_t)stream);
+}
+
+int gd_arnoldi_step_f32(int64_t n, int32_t m, int32_t k, float* V, int64_t ldv, void* ws, double* resid_out,
+                        gd_stream stream)
+{
+  return gd::arnoldi::step<float>(n, m, k, V, ldv, ws, resid_out, (hipStream_t)stream);
+}
+
+int gd_gmres_update_f64(int64_t n, int32_t m, int32_t k, const double* V, int64_t ldv, const void* ws, double* x,
+                        gd_stream stream)
+{
+  return gd::arnoldi::update<double>(n, m, k, V, ldv, ws, x, (hipStream_t)stream);
+}
+
+int gd_gmres_update_f32(int64_t n, int32_t m, int32_t k, const float* V, int64_t ldv, const void* ws, double* x,
+                        gd_stream stream)
+{
+  return gd::arnoldi::update<float>(n, m, k, V, ldv, ws, x, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,10 +1,8 @@
 // Vector kernels for the device-resident Krylov loops.  All reductions
 // accumulate in FP64 and are deterministic (fixed two-stage order).
 //
-// The Gram-Schmidt kernels work on a Krylov basis V stored as k vectors of
-// length n with leading dimension ldv, in FP64 or FP32, each stored vector
-// carrying a scale s_j kept by the host (V_j,true = s_j V_j,stored), so the
-// basis never needs a separate normalisation pass.
+// The multi-vector kernels work on k vectors of length n stored with leading
+// dimension ldv, in FP64 or FP32.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -32,11 +30,12 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[K], int k, doub
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    if (j >= k) break;
-    double v = acc[j];
+    if (j < k) {
+      double v = acc[j];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) red[wv][j] = v;
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) red[wv][j] = v;
+    }
   }
   __syncthreads();
   for (int j = threadIdx.x; j < k; j += kThreads) {
@@ -100,29 +99,31 @@ __global__ void __launch_bounds__(kThreads) k_update(int64_t n, int k, const dou
   else if (mode == 1) block_reduce_store<kMaxK>(acc, 1, partial + (int64_t)blockIdx.x * kMaxK);
 }
 
+// Sum of the nblk block partials of column j, by the whole block in a fixed
+// order (strided partial sums, then a fixed-shape tree), result in thread 0.
+__device__ __forceinline__ double block_sum_column(int nblk, int j, const double* __restrict__ partial)
+{
+  __shared__ double red[kThreads];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kThreads) s += partial[(int64_t)b * kMaxK + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
 // result[j] = sum_b partial[b][j]   (single block, fixed order)
 __global__ void __launch_bounds__(kThreads) k_finalize(int nblk, int k, const double* __restrict__ partial,
                                                        double* __restrict__ result)
 {
-  for (int j = threadIdx.x; j < k; j += kThreads) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * kMaxK + j];
-    result[j] = s;
-  }
-}
-
-// Gram-Schmidt finalize: h[j] = s[j] * sum_b partial[b][j] (true coefficient),
-// c[j] = s[j] * h[j] (coefficient on the stored vector), hcol[j] += h[j].
-__global__ void __launch_bounds__(kThreads) k_gs_finalize(int nblk, int k, const double* __restrict__ partial,
-                                                          const double* __restrict__ s, double* __restrict__ c,
-                                                          double* __restrict__ hcol, int accumulate)
-{
-  for (int j = threadIdx.x; j < k; j += kThreads) {
-    double sum = 0.0;
-    for (int b = 0; b < nblk; ++b) sum += partial[(int64_t)b * kMaxK + j];
-    const double h = s[j] * sum;
-    c[j] = s[j] * h;
-    hcol[j] = accumulate ? hcol[j] + h : h;
+  for (int j = 0; j < k; ++j) {
+    const double s = block_sum_column(nblk, j, partial);
+    if (threadIdx.x == 0) result[j] = s;
   }
 }
 
@@ -189,45 +190,6 @@ int multiaxpy(int64_t n, int k, const double* h, const TV* V, int64_t ldv, TW* w
 
 }  // namespace blas
 
-// --- Gram-Schmidt step used by the solvers (declared in gd_krylov.hpp) ----
-// One classical Gram-Schmidt pass against k <= 32 stored vectors, fused:
-//   pass 1: hcol = S V^T w                     (S = diag(s))
-//   pass 2: w -= V (S hcol); h2 = S V^T w; hcol += h2   (re-orthogonalisation)
-//   pass 3: w -= V (S h2);  norm2 = ||w||^2
-// Device arrays: s[k], c[k] (scratch), hcol[k], norm2[1].
-template <typename TV>
-int cgs2_step(int64_t n, int k, const TV* V, int64_t ldv, TV* w, const double* s, double* c, double* hcol,
-              double* norm2, hipStream_t st)
-{
-  using namespace blas;
-  if (k > kMaxK) return fail(GD_ERR_INVALID, "cgs2_step: at most 32 basis vectors per pass");
-  const int nb = grid_for(n);
-  double* part = scratch(nb);
-  if (!part) return fail(GD_ERR_ALLOC, "cgs2_step: scratch allocation failed");
-  hipLaunchKernelGGL((k_multidot<TV, TV>), dim3(nb), dim3(kThreads), 0, st, n, k, V, ldv, w, part);
-  hipLaunchKernelGGL(k_gs_finalize, dim3(1), dim3(kThreads), 0, st, nb, k, part, s, c, hcol, 0);
-  hipLaunchKernelGGL((k_update<TV, TV>), dim3(nb), dim3(kThreads), 0, st, n, k, c, V, ldv, w, 0, 1.0, part);
-  hipLaunchKernelGGL(k_gs_finalize, dim3(1), dim3(kThreads), 0, st, nb, k, part, s, c, hcol, 1);
-  hipLaunchKernelGGL((k_update<TV, TV>), dim3(nb), dim3(kThreads), 0, st, n, k, c, V, ldv, w, 1, 1.0, part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kThreads), 0, st, nb, 1, part, norm2);
-  GD_HIP(hipGetLastError());
-  return GD_OK;
-}
-
-template int cgs2_step<double>(int64_t, int, const double*, int64_t, double*, const double*, double*, double*,
-                               double*, hipStream_t);
-template int cgs2_step<float>(int64_t, int, const float*, int64_t, float*, const double*, double*, double*,
-                              double*, hipStream_t);
-
-// x += sum_j c_j V_j (x FP64, V FP64 or FP32)
-template <typename TV>
-int basis_update(int64_t n, int k, const double* c, const TV* V, int64_t ldv, double* x, hipStream_t st)
-{
-  return blas::multiaxpy<TV, double>(n, k, c, V, ldv, x, -1.0, st);
-}
-template int basis_update<double>(int64_t, int, const double*, const double*, int64_t, double*, hipStream_t);
-template int basis_update<float>(int64_t, int, const double*, const float*, int64_t, double*, hipStream_t);
-
 }  // namespace gd
 
 using gd::fail;
@@ -292,30 +254,6 @@ int gd_convert_f32_to_f64(int64_t n, double scale, const float* x, double* y, gd
                      dim3(gd::blas::kThreads), 0, (hipStream_t)stream, n, scale, x, y);
   GD_HIP(hipGetLastError());
   return GD_OK;
-}
-
-int gd_cgs2_step_f64(int64_t n, int32_t k, const double* V, int64_t ldv, double* w, const double* s, double* c,
-                     double* hcol, double* norm2, gd_stream stream)
-{
-  return gd::cgs2_step<double>(n, k, V, ldv, w, s, c, hcol, norm2, (hipStream_t)stream);
-}
-
-int gd_cgs2_step_f32(int64_t n, int32_t k, const float* V, int64_t ldv, float* w, const double* s, double* c,
-                     double* hcol, double* norm2, gd_stream stream)
-{
-  return gd::cgs2_step<float>(n, k, V, ldv, w, s, c, hcol, norm2, (hipStream_t)stream);
-}
-
-int gd_basis_update_f64(int64_t n, int32_t k, const double* c, const double* V, int64_t ldv, double* x,
-                        gd_stream stream)
-{
-  return gd::basis_update<double>(n, k, c, V, ldv, x, (hipStream_t)stream);
-}
-
-int gd_basis_update_f32(int64_t n, int32_t k, const double* c, const float* V, int64_t ldv, double* x,
-                        gd_stream stream)
-{
-  return gd::basis_update<float>(n, k, c, V, ldv, x, (hipStream_t)stream);
 }
 
 int gd_add_f32_to_f64(int64_t n, double scale, const float* x, double* y, gd_stream stream)

@@ -319,7 +319,6 @@ __global__ void __launch_bounds__(NT) k3d_apply(K3Params<T> p, const T* __restri
       for (int cc = 0; cc < PL; ++cc)
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
-          const int c = h * PL + cc;
           dot[cc][b] = f0.template atc<LANES, PL>(kn, b, h, cc) * nrm;
           any_pos |= dot[cc][b] > T(0);
           any_np |= !(dot[cc][b] > T(0));
@@ -429,7 +428,6 @@ __global__ void __launch_bounds__(NT) k3d_apply(K3Params<T> p, const T* __restri
       for (int cc = 0; cc < PL; ++cc)
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
-          const int c = h * PL + cc;
           dot[cc][a] = f1.template atc<LANES, PL>(a, kn, h, cc) * nrm;
           any_pos |= dot[cc][a] > T(0);
           any_np |= !(dot[cc][a] > T(0));

@@ -591,4 +591,45 @@ int gd_device_sync(void)
   return GD_OK;
 }
 
+int gd_host_alloc(void** ptr, size_t bytes)
+{
+  if (!ptr) return fail(GD_ERR_INVALID, "gd_host_alloc: null");
+  if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess)
+    return fail(GD_ERR_ALLOC, "gd_host_alloc: hipHostMalloc failed");
+  return GD_OK;
+}
+
+int gd_host_free(void* ptr)
+{
+  GD_HIP(hipHostFree(ptr));
+  return GD_OK;
+}
+
+int gd_event_create(gd_event* ev)
+{
+  if (!ev) return fail(GD_ERR_INVALID, "gd_event_create: null");
+  hipEvent_t e = nullptr;
+  GD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  *ev = (gd_event)e;
+  return GD_OK;
+}
+
+int gd_event_destroy(gd_event ev)
+{
+  GD_HIP(hipEventDestroy((hipEvent_t)ev));
+  return GD_OK;
+}
+
+int gd_event_record(gd_event ev, gd_stream stream)
+{
+  GD_HIP(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+  return GD_OK;
+}
+
+int gd_event_sync(gd_event ev)
+{
+  GD_HIP(hipEventSynchronize((hipEvent_t)ev));
+  return GD_OK;
+}
+
 }  // extern "C"

@@ -184,21 +184,41 @@ int gd_convert_f64_to_f32(int64_t n, double scale, const double* x, float* y, gd
 int gd_convert_f32_to_f64(int64_t n, double scale, const float* x, double* y, gd_stream stream);
 /* y += scale * x (x f32, y f64) */
 int gd_add_f32_to_f64(int64_t n, double scale, const float* x, double* y, gd_stream stream);
-/* One fused classical Gram-Schmidt step with re-orthogonalisation (CGS2)
- * of w against k <= 32 stored basis vectors V_j (true vector s_j V_j):
- * hcol[j] = <s_j V_j, w> summed over both passes, w orthogonalised in place,
- * norm2[0] = ||w||^2 afterwards.  s, c (scratch, k), hcol (k), norm2 (1) are
- * device arrays.  Replaces the sequential modified Gram-Schmidt of
- * gmres.hpp:160-165 (m+1 dot/axpy pairs) by three passes over V. */
-int gd_cgs2_step_f64(int64_t n, int32_t k, const double* V, int64_t ldv, double* w, const double* s,
-                     double* c, double* hcol, double* norm2, gd_stream stream);
-int gd_cgs2_step_f32(int64_t n, int32_t k, const float* V, int64_t ldv, float* w, const double* s,
-                     double* c, double* hcol, double* norm2, gd_stream stream);
-/* x += sum_j c[j] V_j (x FP64; V FP64 or FP32; c device, FP64) */
-int gd_basis_update_f64(int64_t n, int32_t k, const double* c, const double* V, int64_t ldv, double* x,
+/* ---- device-resident Arnoldi process for GMRES(m), m <= 32 ------------ */
+/* Replaces the host-driven modified Gram-Schmidt and Givens loop of
+ * GMRES_no_alloc (gmres.hpp:150-215, m+1 dot/axpy pairs per iteration with a
+ * blocking device-to-host copy per dot) by three fused passes over the basis
+ * per iteration (classical Gram-Schmidt applied twice, FP64 accumulation)
+ * whose last block also forms the Hessenberg column and its Givens rotation.
+ * The workspace (gd_gmres_workspace_bytes(), device memory) holds H, the
+ * rotations, g, the basis scales and the reduction partials.
+ *   gd_gmres_begin:  g = beta e_0, scale of V_0 = s0.
+ *   gd_arnoldi_step: V + (k+1)*ldv holds K V_k on entry and the next basis
+ *     vector on exit; if resid_out is not null, |g_{k+1}| (the GMRES residual
+ *     estimate) is written there by the device (pinned host memory from
+ *     gd_host_alloc is allowed).
+ *   gd_gmres_update: x += V_{0..k-1} y with y from the k x k triangular
+ *     system (gmres.hpp:240-265). */
+size_t gd_gmres_workspace_bytes(void);
+int gd_gmres_begin(void* ws, double beta, double s0, gd_stream stream);
+int gd_arnoldi_step_f64(int64_t n, int32_t m, int32_t k, double* V, int64_t ldv, void* ws, double* resid_out,
                         gd_stream stream);
-int gd_basis_update_f32(int64_t n, int32_t k, const double* c, const float* V, int64_t ldv, double* x,
+int gd_arnoldi_step_f32(int64_t n, int32_t m, int32_t k, float* V, int64_t ldv, void* ws, double* resid_out,
                         gd_stream stream);
+int gd_gmres_update_f64(int64_t n, int32_t m, int32_t k, const double* V, int64_t ldv, const void* ws, double* x,
+                        gd_stream stream);
+int gd_gmres_update_f32(int64_t n, int32_t m, int32_t k, const float* V, int64_t ldv, const void* ws, double* x,
+                        gd_stream stream);
+
+/* Pinned host memory and events, so a host loop can read device results
+ * without draining the stream. */
+typedef void* gd_event;
+int gd_host_alloc(void** ptr, size_t bytes);
+int gd_host_free(void* ptr);
+int gd_event_create(gd_event* ev);
+int gd_event_destroy(gd_event ev);
+int gd_event_record(gd_event ev, gd_stream stream);
+int gd_event_sync(gd_event ev);
 
 /* ---- Krylov solvers on the operator mass_coef*M + adv_coef*A ------------ */
 typedef struct gd_solve_params {
@@ -218,7 +238,8 @@ typedef struct gd_solve_result {
 } gd_solve_result;
 
 /* Restarted GMRES, all FP64 (GMRES_no_alloc semantics, gmres.hpp:74-288,
- * with classical Gram-Schmidt applied twice instead of MGS). x holds x0. */
+ * with the device-resident Arnoldi process above instead of host MGS).
+ * x holds x0. */
 int gd_gmres_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
                  double* x, const gd_solve_params* params, gd_solve_result* result,
                  gd_stream stream);

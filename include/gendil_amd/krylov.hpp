@@ -3,13 +3,16 @@
 // gmres_solve<TV>: restarted GMRES(m) in the shape of GenDiL's GMRES_no_alloc
 // (include/gendil/Solvers/gmres.hpp:74-288): every cycle starts from the true
 // residual r = b - K x computed in FP64, builds an Arnoldi basis of at most m
-// vectors, applies Givens rotations to the Hessenberg column on the host
+// vectors, applies Givens rotations to the Hessenberg column
 // (gmres.hpp:170-210), and ends with x += V y (gmres.hpp:265).  Differences
-// from the reference, all on the device side:
-//   * orthogonalisation is one fused CGS2 step per iteration (three passes
-//     over the basis) instead of m+1 sequential MGS dot/axpy pairs
-//     (gmres.hpp:160-165); every inner product accumulates in FP64;
-//   * basis vectors are stored unnormalised with a host-side scale;
+// from the reference:
+//   * the Arnoldi process runs on the device (gd_arnoldi_step_*): one fused
+//     CGS2 orthogonalisation per iteration (three passes over the basis)
+//     instead of m+1 sequential MGS dot/axpy pairs (gmres.hpp:160-165), every
+//     inner product accumulated in FP64, and the Hessenberg column, Givens
+//     rotation and triangular solve done on the device too;
+//   * the host reads one residual estimate per iteration from pinned memory,
+//     one iteration behind the device, so the stream never drains;
 //   * TV = float stores the basis in FP32 and applies the operator in FP32,
 //     the cycle's starting residual, the inner products, the Hessenberg /
 //     Givens arithmetic and the update of x stay FP64.  Each cycle is then
@@ -17,6 +20,7 @@
 //     Krylov space built on r_k = b - K x_k (FP64).
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -114,6 +118,40 @@ struct SolveReport {
   double seconds = 0.0;
 };
 
+// Pinned host array (the device writes per-iteration results into it).
+template <typename T>
+class PinnedArray {
+ public:
+  explicit PinnedArray(size_t n) { check(gd_host_alloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "gd_host_alloc"); }
+  ~PinnedArray()
+  {
+    if (p_) (void)gd_host_free(p_);
+  }
+  PinnedArray(const PinnedArray&) = delete;
+  PinnedArray& operator=(const PinnedArray&) = delete;
+  T& operator[](size_t i) { return p_[i]; }
+  T* data() { return p_; }
+
+ private:
+  T* p_ = nullptr;
+};
+
+class Event {
+ public:
+  Event() { check(gd_event_create(&e_), "gd_event_create"); }
+  ~Event()
+  {
+    if (e_) (void)gd_event_destroy(e_);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  void record(gd_stream s) { check(gd_event_record(e_, s), "gd_event_record"); }
+  void sync() { check(gd_event_sync(e_), "gd_event_sync"); }
+
+ private:
+  gd_event e_ = nullptr;
+};
+
 namespace detail {
 
 inline double dev_dot(int64_t n, const double* a, const double* b, double* scratch, gd_stream s)
@@ -125,23 +163,28 @@ inline double dev_dot(int64_t n, const double* a, const double* b, double* scrat
 }
 
 template <typename TV>
-int cgs2(int64_t n, int k, const TV* V, int64_t ldv, TV* w, const double* s, double* c, double* hcol,
-         double* norm2, gd_stream st)
+int arnoldi_step(int64_t n, int m, int k, TV* V, void* ws, double* resid, gd_stream st)
 {
-  if constexpr (std::is_same_v<TV, double>) return gd_cgs2_step_f64(n, k, V, ldv, w, s, c, hcol, norm2, st);
-  else return gd_cgs2_step_f32(n, k, V, ldv, w, s, c, hcol, norm2, st);
+  if constexpr (std::is_same_v<TV, double>) return gd_arnoldi_step_f64(n, m, k, V, n, ws, resid, st);
+  else return gd_arnoldi_step_f32(n, m, k, V, n, ws, resid, st);
 }
 
 template <typename TV>
-int basis_update(int64_t n, int k, const double* c, const TV* V, int64_t ldv, double* x, gd_stream st)
+int update(int64_t n, int m, int k, const TV* V, const void* ws, double* x, gd_stream st)
 {
-  if constexpr (std::is_same_v<TV, double>) return gd_basis_update_f64(n, k, c, V, ldv, x, st);
-  else return gd_basis_update_f32(n, k, c, V, ldv, x, st);
+  if constexpr (std::is_same_v<TV, double>) return gd_gmres_update_f64(n, m, k, V, n, ws, x, st);
+  else return gd_gmres_update_f32(n, m, k, V, n, ws, x, st);
 }
 
 }  // namespace detail
 
 // Solve K x = b.  b, x: FP64 device vectors of length n; x holds x0.
+//
+// The host never waits inside a cycle for the iteration it has just queued:
+// iteration k is enqueued before the residual estimate of iteration k-1 is
+// read (from pinned memory, after that iteration's event), so the stream
+// always holds work.  When iteration k-1 meets the tolerance, the speculative
+// iteration k is discarded (its column is not used by the update).
 template <typename TV>
 SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, double* x, const SolveOptions& opt)
 {
@@ -150,17 +193,13 @@ SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, d
   const int m = opt.restart;
   const gd_stream st = K.stream;
   DeviceArray<TV> V((size_t)(m + 1) * n);
-  DeviceArray<double> r(n);
-  DeviceArray<double> small(4 * (m + 2) + 8);  // s | c | hcol | norm2 | dot scratch
-  double* d_s = small.data();
-  double* d_c = d_s + (m + 2);
-  double* d_h = d_c + (m + 2);
-  double* d_n2 = d_h + (m + 2);
-  double* d_dot = d_n2 + 2;
+  DeviceArray<double> r(n), dot(2);
+  DeviceArray<unsigned char> ws(gd_gmres_workspace_bytes());
+  PinnedArray<double> resid(m);
+  std::vector<Event> done(m);
 
-  std::vector<double> s(m + 1, 1.0), H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m), hcol(m + 2);
   SolveReport rep;
-  const double bnorm = std::sqrt(detail::dev_dot(n, b, b, d_dot, st));
+  const double bnorm = std::sqrt(detail::dev_dot(n, b, b, dot.data(), st));
   if (bnorm == 0.0) {  // zero right-hand side: x = 0 (gmres.hpp:106-113)
     check(gd_axpby_f64(n, 0.0, b, 0.0, x, st), "gd_axpby_f64");
     check(gd_stream_sync(st), "sync");
@@ -171,7 +210,7 @@ SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, d
     // true residual in FP64 (gmres.hpp:98-104, 268-275)
     K.apply(1.0, x, r.data());
     check(gd_axpby_f64(n, 1.0, b, -1.0, r.data(), st), "gd_axpby_f64");
-    const double beta = std::sqrt(detail::dev_dot(n, r.data(), r.data(), d_dot, st));
+    const double beta = std::sqrt(detail::dev_dot(n, r.data(), r.data(), dot.data(), st));
     rep.final_rel_residual = beta / bnorm;
     if (rep.final_rel_residual <= opt.rel_tol) {
       rep.success = true;
@@ -179,57 +218,33 @@ SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, d
     }
     if (rep.iterations >= opt.max_iters || rep.cycles >= opt.max_refinements) break;
     ++rep.cycles;
-    // V_0 = r (stored unnormalised, scale 1/beta)
-    if constexpr (std::is_same_v<TV, double>) check(gd_axpby_f64(n, 1.0, r.data(), 0.0, V.data(), st), "copy");
+    // V_0 = r / beta
+    if constexpr (std::is_same_v<TV, double>) check(gd_axpby_f64(n, 1.0 / beta, r.data(), 0.0, V.data(), st), "scale");
     else check(gd_convert_f64_to_f32(n, 1.0 / beta, r.data(), V.data(), st), "convert");
-    s[0] = std::is_same_v<TV, double> ? 1.0 / beta : 1.0;
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
+    check(gd_gmres_begin(ws.data(), beta, 1.0, st), "gd_gmres_begin");
     double target = opt.rel_tol * bnorm;
     if constexpr (!std::is_same_v<TV, double>) target = std::max(target, opt.inner_rel_tol * beta);
-    int k = 0;
-    for (; k < m && rep.iterations < opt.max_iters; ++k) {
-      ++rep.iterations;
-      TV* w = V.data() + (size_t)(k + 1) * n;
-      K.apply(s[k], V.data() + (size_t)k * n, w);
-      check(gd_memcpy_h2d(d_s, s.data(), (k + 1) * sizeof(double)), "h2d");
-      check(detail::cgs2<TV>(n, k + 1, V.data(), n, w, d_s, d_c, d_h, d_n2, st), "cgs2");
-      check(gd_memcpy_d2h(hcol.data(), d_h, (k + 1) * sizeof(double)), "d2h");
-      double n2 = 0.0;
-      check(gd_memcpy_d2h(&n2, d_n2, sizeof(double)), "d2h");
-      const double hk1 = std::sqrt(n2);
-      for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = hcol[i];
-      H[(size_t)(k + 1) * m + k] = hk1;
-      s[k + 1] = hk1 > 0.0 ? 1.0 / hk1 : 0.0;
-      // apply previous rotations, then form the new one (gmres.hpp:170-210)
-      for (int i = 0; i < k; ++i) {
-        const double a = H[(size_t)i * m + k], c2 = H[(size_t)(i + 1) * m + k];
-        H[(size_t)i * m + k] = cs[i] * a + sn[i] * c2;
-        H[(size_t)(i + 1) * m + k] = -sn[i] * a + cs[i] * c2;
-      }
-      const double a = H[(size_t)k * m + k], c2 = H[(size_t)(k + 1) * m + k];
-      const double rho = std::hypot(a, c2);
-      cs[k] = rho > 0.0 ? a / rho : 1.0;
-      sn[k] = rho > 0.0 ? c2 / rho : 0.0;
-      H[(size_t)k * m + k] = rho;
-      H[(size_t)(k + 1) * m + k] = 0.0;
-      g[k + 1] = -sn[k] * g[k];
-      g[k] = cs[k] * g[k];
-      if (std::fabs(g[k + 1]) <= target || hk1 == 0.0) {  // converged or happy breakdown
-        ++k;
-        break;
+    const int budget = std::min(m, opt.max_iters - rep.iterations);
+    int used = budget;  // columns that enter the update
+    for (int k = 0; k < budget; ++k) {
+      TV* vk = V.data() + (size_t)k * n;
+      K.apply(1.0, vk, vk + n);
+      check(detail::arnoldi_step<TV>(n, m, k, V.data(), ws.data(), resid.data() + k, st), "gd_arnoldi_step");
+      done[k].record(st);
+      if (k >= 1) {
+        done[k - 1].sync();
+        const double e = resid[k - 1];
+        if (e <= target) {  // converged (or happy breakdown) at iteration k-1
+          used = k;
+          break;
+        }
       }
     }
-    // back substitution H y = g, then x += V (s y)  (gmres.hpp:240-265)
-    for (int i = k - 1; i >= 0; --i) {
-      double acc = g[i];
-      for (int j = i + 1; j < k; ++j) acc -= H[(size_t)i * m + j] * y[j];
-      y[i] = acc / H[(size_t)i * m + i];
+    if (used == budget && budget > 0) {
+      done[budget - 1].sync();
     }
-    std::vector<double> coef(k);
-    for (int j = 0; j < k; ++j) coef[j] = s[j] * y[j];
-    check(gd_memcpy_h2d(d_c, coef.data(), k * sizeof(double)), "h2d");
-    check(detail::basis_update<TV>(n, k, d_c, V.data(), n, x, st), "basis_update");
+    rep.iterations += used;
+    check(detail::update<TV>(n, m, used, V.data(), ws.data(), x, st), "gd_gmres_update");
   }
   check(gd_stream_sync(st), "sync");
   rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -70,3 +70,67 @@ def test_zero_rhs_gives_zero_solution():
     x = torch.ones_like(b)
     res = S.gmres(K, torch.zeros_like(b), x)
     assert res.success and float(x.abs().max()) == 0.0
+
+
+def test_gmres_converges_inside_first_cycle():
+    # DG mass matrix: block diagonal, well conditioned, so GMRES(30) stops
+    # mid-cycle; this covers the discarded speculative iteration.
+    mesh = G.CartesianMesh.unit(3, 6, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(2))
+    M = G.MakeMassFiniteElementOperator(fes, G.MakeIntegrationRule(3))
+    xt = torch.as_tensor(G.fill_random(M.num_dofs, 3), device="cuda")
+    b = torch.empty_like(xt)
+    M(xt, b)
+    for fn in (S.gmres, S.gmres_mixed):
+        x = torch.zeros_like(b)
+        res = fn(M, b, x, restart=30, rel_tol=1e-11)
+        print(fn.__name__, res, flush=True)
+        assert res.success and res.iterations < 30 * max(res.refinements, 1)
+        assert float((x - xt).norm() / xt.norm()) <= 1e-8
+
+
+@pytest.mark.parametrize("restart", [1, 3, 32])
+def test_gmres_restart_lengths(restart):
+    mesh = G.CartesianMesh.unit(3, 4, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(2))
+    M = G.MakeMassFiniteElementOperator(fes, G.MakeIntegrationRule(3))
+    xt = torch.as_tensor(G.fill_random(M.num_dofs, 4), device="cuda")
+    b = torch.empty_like(xt)
+    M(xt, b)
+    x = torch.zeros_like(b)
+    res = S.gmres(M, b, x, restart=restart, rel_tol=1e-10, max_iters=20000)
+    assert res.success, res
+    assert float((x - xt).norm() / xt.norm()) <= 1e-7
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_arnoldi_basis_is_orthonormal(dtype):
+    # m Arnoldi steps through the C-ABI on the implicit-step operator: the
+    # stored basis is orthonormal to the working precision.
+    from gendil_amd import _lib as L
+
+    K, b, _ = implicit_step(4)
+    n, m = K.num_dofs, 12
+    lib = L.lib()
+    V = torch.zeros((m + 1, n), dtype=dtype, device="cuda")
+    V[0] = (b / b.norm()).to(dtype)
+    ws = torch.zeros(lib.gd_gmres_workspace_bytes(), dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    L.check(lib.gd_gmres_begin(ws.data_ptr(), float(b.norm()), 1.0, st), "begin")
+    resid = torch.zeros(m, dtype=torch.float64, device="cuda")
+    step = lib.gd_arnoldi_step_f64 if dtype == torch.float64 else lib.gd_arnoldi_step_f32
+    for k in range(m):
+        if dtype == torch.float64:
+            K(V[k], V[k + 1])
+        else:
+            L.check(lib.gd_apply_f32(K.plan.handle, 1.0, K.adv_coef, V[k].data_ptr(), V[k + 1].data_ptr(), st), "f32")
+        L.check(step(n, m, k, V.data_ptr(), n, ws.data_ptr(), resid[k:].data_ptr(), st), "step")
+    torch.cuda.synchronize()
+    Vd = V.double()
+    gram = Vd @ Vd.T
+    err = float((gram - torch.eye(m + 1, dtype=torch.float64, device="cuda")).abs().max())
+    tol = 1e-13 if dtype == torch.float64 else 1e-6
+    print(f"{dtype}: max |V^T V - I| = {err:.3e}, residual estimates {resid.tolist()[:4]} ...")
+    assert err <= tol
+    r = resid.cpu().numpy()
+    assert np.all(np.diff(r) <= 1e-12 * r[0])  # GMRES residual estimates never increase
