@@ -10,9 +10,10 @@ Infinity Cache cannot hold the working set.
 
 Prints one JSON line: GDoF/s, ms per apply, the HBM roofline of the apply
 kernel (HIP events on the launch stream), the CPU baseline (GenDiL's own
-OpenMP apply compiled from its headers, on this host), the FP32 apply, and
-the FP64 vs FP32-refined GMRES(30) solve of the implicit step
-(M + dt A) x = M u0 on the 32^3 mesh.
+OpenMP apply compiled from its headers, on this host), the FP32 apply, the
+6D p-adaptive FP32 apply (8^6 elements, orders 1..3, one launch), and the
+FP64 vs FP32-refined GMRES(30) solve of the implicit step (M + dt A) x = M u0
+on the 32^3 mesh.
 """
 from __future__ import annotations
 
@@ -38,6 +39,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES comparison")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-padapt", action="store_true", help="skip the 6D p-adaptive apply")
     return ap.parse_args()
 
 
@@ -143,6 +145,47 @@ def solve_comparison():
     }
 
 
+def padapt_6d(steps, warmup, n=8):
+    """6D phase-space advection, n^6 elements, per-element orders p in {1,2,3}
+    (probabilities 0.5/0.35/0.15, seed 2024), q = p + 1, FP32, one launch per
+    apply; the thread layout comes from the autotuner."""
+    import torch
+
+    import gendil_amd as G
+    from gendil_amd import workloads as W
+
+    fes, adv = W.phase_space_6d_padaptive(n, seed=2024)
+    op = G.MakeAdvectionOperator(fes, None, adv)
+    N = op.num_dofs
+    E = fes.mesh.num_cells
+    chosen, tuned_ms = op.plan.autotune(torch.float32, 0.0, 1.0, reps=3)
+    gen = torch.Generator(device="cuda").manual_seed(2024)
+    x0 = torch.rand(N, generator=gen, device="cuda", dtype=torch.float32) * 2.0 - 1.0
+    xs = [x0, x0 * 0.5, x0 * -0.25]
+    ys = [torch.empty_like(x0) for _ in range(3)]
+    ms = time_applies(op, xs, ys, steps, warmup)
+    t = ms * 1e-3
+    # algorithmic bytes: u read + y written (FP32), per element its order
+    # (1 B), DoF offset (8 B) and bucket entry (4 B)
+    nbytes = 8.0 * N + 13.0 * E
+    y = ys[(steps - 1) % 3].double()
+    cons = float(y.sum().abs() / y.abs().sum())
+    counts = [int((fes.elem_order == p).sum()) for p in (1, 2, 3)]
+    del xs, ys
+    return {
+        "workload": f"6D phase space {n}^6 (space periodic), p in {{1,2,3}} w.p. 0.5/0.35/0.15, q = p+1, "
+        "beta = (X3, X4, X5, 0.1, 0, 0), FP32, one launch",
+        "dofs": N,
+        "elements_per_order": {"p1": counts[0], "p2": counts[1], "p3": counts[2]},
+        "value": N / t / 1e9,
+        "unit": "GDoF/s",
+        "ms_per_step": ms,
+        "roofline_frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
+        "autotuned_variant": chosen,
+        "conservation_sum_over_abs_sum": cons,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -209,6 +252,9 @@ def main():
         del x32, y32
     del xs
     torch.cuda.empty_cache()
+    if not args.no_padapt:
+        result["padapt6d"] = padapt_6d(max(10, args.steps // 4), 3)
+        torch.cuda.empty_cache()
     if not args.no_solve:
         result["solve"] = solve_comparison()
     if not args.no_cpu:

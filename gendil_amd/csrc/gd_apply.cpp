@@ -8,6 +8,7 @@
 
 #include "gd_dispatch.hpp"
 #include "gd_k3d_api.hpp"
+#include "gd_padapt_api.hpp"
 
 namespace gd {
 
@@ -34,7 +35,7 @@ int tuned_lanes(const gd_plan* plan, int /*N*/, size_t bytes) { return plan->tun
 int lookup(const gd_plan* plan, const KernelSet** ks)
 {
   if (!plan) return fail(GD_ERR_INVALID, "null plan");
-  if (plan->padapt) return fail(GD_ERR_UNSUPPORTED, "p-adaptive plans use gd_padapt_apply_*");
+  if (plan->padapt) return fail(GD_ERR_UNSUPPORTED, "p-adaptive plans support the forward apply only");
   const int N = plan->desc.order + 1, Q = plan->desc.num_quad;
   *ks = find_kernels(plan->dim, N, Q);
   if (!*ks)
@@ -73,8 +74,23 @@ bool k3_eligible(const gd_plan* plan, int lanes, bool adv, bool mass)
 }
 
 template <typename T>
+int padapt_apply(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, hipStream_t s)
+{
+  if (!x || !y) return fail(GD_ERR_INVALID, "null vector");
+  if ((const void*)x == (const void*)y) return fail(GD_ERR_INVALID, "input and output must not alias");
+  if (adv_coef != T(0)) {
+    const int rc = check_adv(plan, false);
+    if (rc) return rc;
+  }
+  int v = plan->tuned_pa[sizeof(T) == 8 ? 0 : 1];
+  if (const char* env = std::getenv("GENDIL_AMD_PA_VARIANT")) v = std::atoi(env);
+  return pa::apply<T>(plan, v, mass_coef, adv_coef, x, y, s);
+}
+
+template <typename T>
 int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, hipStream_t s)
 {
+  if (plan && plan->padapt) return padapt_apply<T>(plan, mass_coef, adv_coef, x, y, s);
   const KernelSet* ks = nullptr;
   int rc = lookup(plan, &ks);
   if (rc) return rc;
@@ -163,6 +179,10 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   if (!plan || (precision_bytes != 4 && precision_bytes != 8)) return fail(GD_ERR_INVALID, "autotune: bad arguments");
   if (reps < 1) reps = 5;
   const int slot = precision_bytes == 8 ? 0 : 1;
+  if (plan->padapt) {
+    const char* why = nullptr;
+    if (!gd::pa::eligible(plan, &why)) return fail(GD_ERR_UNSUPPORTED, why);
+  }
   const size_t bytes = (size_t)plan->ndofs * precision_bytes;
   void *x = nullptr, *y = nullptr;
   if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
@@ -173,18 +193,24 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  const int saved = plan->tuned_k3_lanes[slot];
+  int* knob = plan->padapt ? &plan->tuned_pa[slot] : &plan->tuned_k3_lanes[slot];
+  std::vector<int> choices;
+  if (plan->padapt)
+    for (int v = 0; v < gd::pa::num_variants(plan->dim); ++v) choices.push_back(v);
+  else
+    choices = {0, 1, 2, 4};
+  const int saved = *knob;
   double best = 1e300;
   int best_lanes = 0;
   int rc = GD_OK;
-  for (int lanes : {0, 1, 2, 4}) {
-    plan->tuned_k3_lanes[slot] = lanes;
+  for (int lanes : choices) {
+    *knob = lanes;
     auto run = [&]() -> int {
       if (precision_bytes == 8)
         return gd_apply_f64(plan, mass_coef, adv_coef, (const double*)x, (double*)y, nullptr);
       return gd_apply_f32(plan, (float)mass_coef, (float)adv_coef, (const float*)x, (float*)y, nullptr);
     };
-    if (lanes > 0) {
+    if (lanes > 0 && !plan->padapt) {
       const bool ok = precision_bytes == 8
                           ? gd::k3_eligible<double>(plan, lanes, adv_coef != 0.0, mass_coef != 0.0)
                           : gd::k3_eligible<float>(plan, lanes, adv_coef != 0.0, mass_coef != 0.0);
@@ -210,10 +236,10 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   (void)hipFree(x);
   (void)hipFree(y);
   if (rc) {
-    plan->tuned_k3_lanes[slot] = saved;
+    *knob = saved;
     return rc;
   }
-  plan->tuned_k3_lanes[slot] = best_lanes;
+  *knob = best_lanes;
   if (chosen) *chosen = best_lanes;
   if (best_ms) *best_ms = best;
   return GD_OK;
@@ -221,10 +247,17 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
 
 int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes)
 {
-  if (!plan || (precision_bytes != 4 && precision_bytes != 8) || (lanes != -1 && lanes != 0 && lanes != 1 &&
-                                                                    lanes != 2 && lanes != 4))
+  if (!plan || (precision_bytes != 4 && precision_bytes != 8)) return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
+  const int slot = precision_bytes == 8 ? 0 : 1;
+  if (plan->padapt) {
+    if (lanes < -1 || lanes >= gd::pa::num_variants(plan->dim))
+      return fail(GD_ERR_INVALID, "gd_plan_set_variant: no such p-adaptive variant");
+    plan->tuned_pa[slot] = lanes;
+    return GD_OK;
+  }
+  if (lanes != -1 && lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4)
     return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
-  plan->tuned_k3_lanes[precision_bytes == 8 ? 0 : 1] = lanes;
+  plan->tuned_k3_lanes[slot] = lanes;
   return GD_OK;
 }
 

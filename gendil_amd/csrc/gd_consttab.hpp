@@ -160,5 +160,40 @@ constexpr Tables<N, Q> make_gll_gauss()
   return T;
 }
 
+// Collocated derivative on the Q Gauss points: Gq[a][b] = l_b'(x_a), l_b the
+// Lagrange polynomials on those points (antisymmetric under the mirror map).
+template <int Q>
+struct Colloc {
+  double Gq[Q][Q] = {};
+};
+
+template <int Q>
+constexpr Colloc<Q> make_gauss_colloc()
+{
+  const Tables<1, Q> g = make_gll_gauss<1, Q>();
+  Colloc<Q> C;
+  for (int a = 0; a < Q; ++a)
+    for (int k = 0; k < Q; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < Q; ++i) {
+        if (i == k) continue;
+        double nu = 1.0, de = g.xq[k] - g.xq[i];
+        for (int j = 0; j < Q; ++j)
+          if (j != i && j != k) {
+            nu *= g.xq[a] - g.xq[j];
+            de *= g.xq[k] - g.xq[j];
+          }
+        s += nu / de;
+      }
+      C.Gq[a][k] = s;
+    }
+  for (int a = 0; a < Q; ++a)
+    for (int k = 0; k < Q; ++k) {
+      const int am = Q - 1 - a, km = Q - 1 - k;
+      if (a * Q + k < am * Q + km) C.Gq[am][km] = -C.Gq[a][k];
+    }
+  return C;
+}
+
 }  // namespace ct
 }  // namespace gd

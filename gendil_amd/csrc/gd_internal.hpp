@@ -94,6 +94,11 @@ struct HostField {
   std::vector<double> data;
 };
 
+namespace pa {
+struct State;
+void destroy_state(State* s);
+}  // namespace pa
+
 }  // namespace gd
 
 struct gd_plan {
@@ -128,6 +133,12 @@ struct gd_plan {
   // autotuned lanes per element of the register-resident 3D kernel
   // ([0] FP64, [1] FP32; -1 = not tuned, 0 = generic kernel)
   int tuned_k3_lanes[2] = {-1, -1};
+  // p-adaptive path: kernel variant per precision (-1 = default), lazily
+  // built element buckets and device parameter block, and a counter bumped by
+  // every coefficient change so the device copy is refreshed
+  int tuned_pa[2] = {-1, -1};
+  mutable gd::pa::State* pa_state = nullptr;
+  int coef_version = 0;
   ~gd_plan();
 };
 

@@ -1,0 +1,40 @@
+// Host-side interface of the p-adaptive persistent apply (gd_padapt.hpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+struct gd_plan;
+
+namespace gd {
+namespace pa {
+
+// Threads per element for the n = 4, 3, 2 buckets (64 = one wave, 256 = one
+// block); a block of 256 threads holds 256 / tpe elements of one bucket.
+struct Variant {
+  int tpe4, tpe3, tpe2;
+};
+
+int num_variants(int D);
+Variant variant(int D, int v);
+
+// Launch one instantiation.  pp: device PParams<T>; it0/it1: number of work
+// items of the n=4 and n=3 buckets; grid: total items.
+template <typename T>
+int launch(int D, bool adv, int v, int grid, int it0, int it1, const void* pp, T mass_coef, T adv_coef, const T* x,
+           T* y, hipStream_t s);
+
+// Plan-side state (element buckets, device parameter copies).
+struct State;
+void destroy_state(State* s);
+
+// Is the plan served by this path?  On false, *why says what is missing.
+bool eligible(const gd_plan* plan, const char** why);
+
+template <typename T>
+int apply(const gd_plan* plan, int v, T mass_coef, T adv_coef, const T* x, T* y, hipStream_t s);
+
+}  // namespace pa
+}  // namespace gd

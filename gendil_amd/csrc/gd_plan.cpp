@@ -344,6 +344,7 @@ gd_plan::~gd_plan()
   if (elem_order_dev) (void)hipFree(elem_order_dev);
   if (elem_quad_dev) (void)hipFree(elem_quad_dev);
   gd::free_fields(this);
+  gd::pa::destroy_state(pa_state);
 }
 
 using gd::fail;
@@ -494,6 +495,7 @@ int gd_plan_set_sigma(gd_plan* plan, const gd_separable* sigma)
   int rc = gd::check_sep(plan, sigma, "sigma", false);
   if (rc) return rc;
   gd::copy_sep(plan, sigma, plan->sigma_host, 1.0);
+  ++plan->coef_version;
   return gd::upload_fields(plan);
 }
 
@@ -507,6 +509,7 @@ int gd_plan_set_beta_separable(gd_plan* plan, const gd_separable* beta)
   for (int d = 0; d < plan->dim; ++d) gd::copy_sep(plan, &beta[d], plan->beta_host[d], 0.0);
   plan->beta_kind = GD_BETA_SEPARABLE;
   plan->beta_set = true;
+  ++plan->coef_version;
   return gd::upload_fields(plan);
 }
 
@@ -520,6 +523,7 @@ int gd_plan_set_beta_affine(gd_plan* plan, const double* A, const double* c)
   }
   plan->beta_kind = GD_BETA_AFFINE;
   plan->beta_set = true;
+  ++plan->coef_version;
   return GD_OK;
 }
 
@@ -534,6 +538,7 @@ int gd_plan_set_beta_pointwise_f64(gd_plan* plan, const double* beta_cell, const
   }
   plan->beta_kind = GD_BETA_POINTWISE;
   plan->beta_set = true;
+  ++plan->coef_version;
   return GD_OK;
 }
 
@@ -545,6 +550,7 @@ int gd_plan_set_bc(gd_plan* plan, int32_t mode, const gd_separable* g)
   if (rc) return rc;
   plan->bc_mode = mode;
   gd::copy_sep(plan, g, plan->bc_host, 0.0);
+  ++plan->coef_version;
   return gd::upload_fields(plan);
 }
 

@@ -56,6 +56,24 @@ def phase_space_6d_separable() -> VectorField:
     return VectorField(comps)
 
 
+def padaptive_orders(num_elements: int, seed: int = 2024, probs=(0.5, 0.35, 0.15)) -> np.ndarray:
+    """Per-element orders p in {1, 2, 3} drawn with the given probabilities."""
+    rng = np.random.default_rng(seed)
+    return rng.choice(np.array([1, 2, 3], dtype=np.int8), size=num_elements, p=np.asarray(probs))
+
+
+def phase_space_6d_padaptive(n: int = 8, seed: int = 2024):
+    """6D phase-space workload on [0,1]^6, n elements per axis, periodic in
+    the three space axes (no boundary terms in velocity), per-element orders
+    drawn from {1,2,3} with probabilities {0.5, 0.35, 0.15}, q_e = p_e + 1
+    Gauss points.  Returns (space, beta) with beta = (X3, X4, X5, 0.1, 0, 0)."""
+    mesh = CartesianMesh([n] * 6, [1.0 / n] * 6, [0.0] * 6, [True, True, True, False, False, False])
+    orders = padaptive_orders(mesh.num_cells, seed)
+    from .discretization import MakePAdaptiveSpace
+
+    return MakePAdaptiveSpace(mesh, orders, orders + 1), phase_space_6d_affine()
+
+
 def advection_c2_numpy(X: np.ndarray) -> np.ndarray:
     """The benchmark field evaluated at points X[..., 3]."""
     b = np.empty(X.shape[:-1] + (3,))

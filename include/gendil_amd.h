@@ -141,15 +141,19 @@ int gd_adv_grad_beta_f64(const gd_plan* plan, double adv_coef, const double* u,
                          gd_stream stream);
 
 /* ---- kernel variant selection (autotuner) ------------------------------ */
-/* Times every compiled variant of the apply for this plan (the generic
- * LDS-staged kernel, and the register-resident 3D kernel with 1, 2 or 4 lanes
- * per element where it applies) on scratch vectors, keeps the fastest for
- * later applies of that precision (4 or 8 bytes), and reports it (0 = generic
- * kernel).  No GenDiL counterpart: GenDiL fixes its thread layout at compile
- * time (ThreadBlockLayout, threadlayout.hpp:46). */
+/* Times every compiled variant of the apply for this plan on scratch
+ * vectors, keeps the fastest for later applies of that precision (4 or 8
+ * bytes), and reports it.  Uniform plans: the generic LDS-staged kernel (0)
+ * and the register-resident 3D kernel with 1, 2 or 4 lanes per element where
+ * it applies.  p-adaptive plans: the thread-group layouts of the persistent
+ * kernel (6D: 0 = one wave per n=3 element, 1 = one block per n=3 element;
+ * n=4 elements take a block, n=2 elements a wave).  No GenDiL counterpart:
+ * GenDiL fixes its thread layout at compile time (ThreadBlockLayout,
+ * threadlayout.hpp:46). */
 int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, double adv_coef,
                      int32_t reps, int32_t* chosen, double* best_ms);
-/* Pin a variant: lanes = -1 (default heuristic), 0 (generic), 1, 2 or 4. */
+/* Pin a variant: -1 (default); uniform plans 0 (generic), 1, 2 or 4 lanes;
+ * p-adaptive plans the variant index as reported by gd_plan_autotune. */
 int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes);
 
 /* ---- device memory and vector kernels for the Krylov loops ------------ */
