@@ -2,14 +2,14 @@
 //
 // One iteration k (basis vectors V_0..V_k stored with scales s_j, true vector
 // v_j = s_j V_j, and W = V_{k+1} holding K V_k) is three fused passes over the
-// basis, each ending in a deterministic reduction done by the last block to
-// finish (block partials summed in block order):
+// basis, each followed by a one-block kernel that sums the block partials in
+// block order (deterministic):
 //   pass A: h1_j = s_j <V_j, W>                         (classical GS)
 //   pass B: W -= sum_j s_j h1_j V_j, then h2_j = s_j <V_j, W>, ||W||^2
 //           (re-orthogonalisation, the "twice is enough" CGS2 step)
 //   pass C: Z = (W - sum_j s_j h2_j V_j) / p with p^2 = ||W||^2 - sum_j h2_j^2
 //           (Pythagoras), stored as V_{k+1}; nu = ||Z|| exactly; s_{k+1} = 1/nu
-// and the last block of pass C finishes the Hessenberg column
+// and the finalize kernel of pass C finishes the Hessenberg column
 // H_{jk} = s_k (h1_j + h2_j), H_{k+1,k} = s_k p nu, applies the previous Givens
 // rotations and forms the new one (gmres.hpp:170-210), and writes the residual
 // estimate |g_{k+1}| to an optional caller pointer (pinned host memory, so the
@@ -42,7 +42,6 @@ struct Workspace {
   double c[kMaxM];   // coefficients on the stored vectors for the next update pass
   double h1[kMaxM];  // accumulated true GS coefficients of the current column
   double pnorm, pinv;
-  unsigned int counter[4];
   double partial[kMaxBlocks][kStride];
 };
 
@@ -81,25 +80,8 @@ __device__ __forceinline__ void store_block_partials(const double (&acc)[K], int
   }
 }
 
-// True in the one block that finishes last; its reads of the other blocks'
-// partials are ordered after their writes (release/acquire at device scope).
-// The last block resets the counter for the next launch.
-__device__ __forceinline__ bool last_block(unsigned int* counter)
-{
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    if (threadIdx.x == 0) *counter = 0u;
-  }
-  return is_last;
-}
-
 // out[j] = sum over blocks of partial[b][j], j < cols: one wave per column,
-// fixed order.  Results land in shared memory `col` (all threads see them
+// fixed order.  Results land in shared memory `col` (visible to all threads
 // after the trailing barrier).
 __device__ __forceinline__ void sum_columns(const Workspace* ws, int nblk, int cols, double* col)
 {
@@ -113,26 +95,33 @@ __device__ __forceinline__ void sum_columns(const Workspace* ws, int nblk, int c
   __syncthreads();
 }
 
-// Pass A.
-template <typename TV>
-__global__ void __launch_bounds__(kThreads) k_pass_a(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
+// The basis passes take the number of stored vectors KK as a template
+// parameter: every load of a pass is then unconditional and independent, so
+// all of them are in flight together (a runtime count made each load a
+// predicated branch followed by a wait).  Reductions end in a single-block
+// finalize kernel; the kernel boundary orders the partials.
+
+// Pass A: partial[b][j] = <V_j, W> over block b's rows.
+template <typename TV, int KK>
+__global__ void __launch_bounds__(kThreads) k_pass_a(int64_t n, const TV* __restrict__ V, int64_t ldv,
                                                      Workspace* __restrict__ ws)
 {
-  const int kk = k + 1;
-  const TV* __restrict__ w = V + (int64_t)(k + 1) * ldv;
-  double acc[kMaxM];
+  const TV* __restrict__ w = V + (int64_t)KK * ldv;
+  double acc[KK];
 #pragma unroll
-  for (int j = 0; j < kMaxM; ++j) acc[j] = 0.0;
+  for (int j = 0; j < KK; ++j) acc[j] = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
     const double wi = (double)w[i];
 #pragma unroll
-    for (int j = 0; j < kMaxM; ++j)
-      if (j < kk) acc[j] += (double)V[j * ldv + i] * wi;
+    for (int j = 0; j < KK; ++j) acc[j] += (double)V[j * ldv + i] * wi;
   }
-  store_block_partials<kMaxM>(acc, kk, ws->partial[blockIdx.x]);
-  if (!last_block(&ws->counter[0])) return;
+  store_block_partials<KK>(acc, KK, ws->partial[blockIdx.x]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_fin_a(int nblk, int kk, Workspace* __restrict__ ws)
+{
   __shared__ double col[kStride];
-  sum_columns(ws, gridDim.x, kk, col);
+  sum_columns(ws, nblk, kk, col);
   for (int j = threadIdx.x; j < kk; j += kThreads) {
     const double h = ws->s[j] * col[j];
     ws->h1[j] = h;
@@ -140,42 +129,40 @@ __global__ void __launch_bounds__(kThreads) k_pass_a(int64_t n, int k, const TV*
   }
 }
 
-// Pass B.
-template <typename TV>
-__global__ void __launch_bounds__(kThreads) k_pass_b(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
+// Pass B: W -= sum_j c_j V_j; partial = <V_j, W_new>, ||W_new||^2 (column KK).
+template <typename TV, int KK>
+__global__ void __launch_bounds__(kThreads) k_pass_b(int64_t n, TV* __restrict__ V, int64_t ldv,
                                                      Workspace* __restrict__ ws)
 {
-  const int kk = k + 1;
-  TV* __restrict__ w = const_cast<TV*>(V) + (int64_t)(k + 1) * ldv;
-  double cj[kMaxM];
+  TV* __restrict__ w = V + (int64_t)KK * ldv;
+  double cj[KK];
 #pragma unroll
-  for (int j = 0; j < kMaxM; ++j) cj[j] = j < kk ? ws->c[j] : 0.0;
-  double acc[kMaxM + 1];
+  for (int j = 0; j < KK; ++j) cj[j] = ws->c[j];
+  double acc[KK + 1];
 #pragma unroll
-  for (int j = 0; j <= kMaxM; ++j) acc[j] = 0.0;
+  for (int j = 0; j <= KK; ++j) acc[j] = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-    double v[kMaxM];
+    double v[KK];
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < kMaxM; ++j) {
-      v[j] = j < kk ? (double)V[j * ldv + i] : 0.0;
+    for (int j = 0; j < KK; ++j) {
+      v[j] = (double)V[j * ldv + i];
       s += cj[j] * v[j];
     }
     const TV wt = (TV)((double)w[i] - s);
     w[i] = wt;
     const double wr = (double)wt;
 #pragma unroll
-    for (int j = 0; j < kMaxM; ++j) acc[j] += v[j] * wr;
-    acc[kMaxM] += wr * wr;
+    for (int j = 0; j < KK; ++j) acc[j] += v[j] * wr;
+    acc[KK] += wr * wr;
   }
-  // move the norm accumulator to column kk so the partial row is dense
-#pragma unroll
-  for (int j = 0; j < kMaxM; ++j)
-    if (j == kk) acc[j] = acc[kMaxM];
-  store_block_partials<kMaxM + 1>(acc, kk + 1, ws->partial[blockIdx.x]);
-  if (!last_block(&ws->counter[1])) return;
+  store_block_partials<KK + 1>(acc, KK + 1, ws->partial[blockIdx.x]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_fin_b(int nblk, int kk, Workspace* __restrict__ ws)
+{
   __shared__ double col[kStride];
-  sum_columns(ws, gridDim.x, kk + 1, col);
+  sum_columns(ws, nblk, kk + 1, col);
   if (threadIdx.x == 0) {
     double ss = 0.0;
     for (int j = 0; j < kk; ++j) {
@@ -191,33 +178,37 @@ __global__ void __launch_bounds__(kThreads) k_pass_b(int64_t n, int k, const TV*
   }
 }
 
-// Pass C, with the Hessenberg column and its Givens rotation.
-template <typename TV>
-__global__ void __launch_bounds__(kThreads) k_pass_c(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
-                                                     Workspace* __restrict__ ws, double* __restrict__ resid_out)
+// Pass C: Z = (W - sum_j c_j V_j) / p stored as V_KK; partial = ||Z||^2.
+template <typename TV, int KK>
+__global__ void __launch_bounds__(kThreads) k_pass_c(int64_t n, TV* __restrict__ V, int64_t ldv,
+                                                     Workspace* __restrict__ ws)
 {
-  const int kk = k + 1;
-  TV* __restrict__ w = const_cast<TV*>(V) + (int64_t)(k + 1) * ldv;
+  TV* __restrict__ w = V + (int64_t)KK * ldv;
   const double pinv = ws->pinv;
-  double cj[kMaxM];
+  double cj[KK];
 #pragma unroll
-  for (int j = 0; j < kMaxM; ++j) cj[j] = j < kk ? ws->c[j] : 0.0;
+  for (int j = 0; j < KK; ++j) cj[j] = ws->c[j];
   double acc[1] = {0.0};
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < kMaxM; ++j)
-      if (j < kk) s += cj[j] * (double)V[j * ldv + i];
+    for (int j = 0; j < KK; ++j) s += cj[j] * (double)V[j * ldv + i];
     const TV zt = (TV)(((double)w[i] - s) * pinv);
     w[i] = zt;
     const double zr = (double)zt;
     acc[0] += zr * zr;
   }
   store_block_partials<1>(acc, 1, ws->partial[blockIdx.x]);
-  if (!last_block(&ws->counter[2])) return;
+}
+
+// Finishes column k of the Hessenberg matrix and its Givens rotation.
+__global__ void __launch_bounds__(kThreads) k_fin_c(int nblk, int k, Workspace* __restrict__ ws,
+                                                    double* __restrict__ resid_out)
+{
   __shared__ double col[kStride];
-  sum_columns(ws, gridDim.x, 1, col);
+  sum_columns(ws, nblk, 1, col);
   if (threadIdx.x == 0) {
+    const int kk = k + 1;
     const double nu = sqrt(col[0]);
     const double sk = ws->s[k];
     ws->s[k + 1] = nu > 0.0 ? 1.0 / nu : 0.0;
@@ -243,40 +234,51 @@ __global__ void __launch_bounds__(kThreads) k_pass_c(int64_t n, int k, const TV*
   }
 }
 
-// Cycle start: g = beta e_0, s_0 = s0, counters cleared.
+// Cycle start: g = beta e_0, s_0 = s0.
 __global__ void k_begin(double beta, double s0, Workspace* __restrict__ ws)
 {
   const int t = threadIdx.x;
   if (t <= kMaxM) ws->g[t] = t == 0 ? beta : 0.0;
   if (t == 0) ws->s[0] = s0;
-  if (t < 4) ws->counter[t] = 0u;
 }
 
 // Cycle end: y = H(0:k,0:k)^{-1} g(0:k) in every block, x += sum_j s_j y_j V_j.
-template <typename TV>
-__global__ void __launch_bounds__(kThreads) k_update_x(int64_t n, int k, const TV* __restrict__ V, int64_t ldv,
+template <typename TV, int KK>
+__global__ void __launch_bounds__(kThreads) k_update_x(int64_t n, const TV* __restrict__ V, int64_t ldv,
                                                        const Workspace* __restrict__ ws, double* __restrict__ x)
 {
-  __shared__ double coef[kMaxM];
+  __shared__ double coef[KK];
   if (threadIdx.x == 0) {
-    double y[kMaxM];
-    for (int i = k - 1; i >= 0; --i) {
+    double y[KK];
+#pragma unroll
+    for (int i = KK - 1; i >= 0; --i) {
       double acc = ws->g[i];
-      for (int j = i + 1; j < k; ++j) acc -= ws->H[j][i] * y[j];
+#pragma unroll
+      for (int j = i + 1; j < KK; ++j) acc -= ws->H[j][i] * y[j];
       y[i] = acc / ws->H[i][i];
     }
-    for (int j = 0; j < k; ++j) coef[j] = ws->s[j] * y[j];
+#pragma unroll
+    for (int j = 0; j < KK; ++j) coef[j] = ws->s[j] * y[j];
   }
   __syncthreads();
-  double cj[kMaxM];
+  double cj[KK];
 #pragma unroll
-  for (int j = 0; j < kMaxM; ++j) cj[j] = j < k ? coef[j] : 0.0;
+  for (int j = 0; j < KK; ++j) cj[j] = coef[j];
   for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < kMaxM; ++j)
-      if (j < k) s += cj[j] * (double)V[j * ldv + i];
+    for (int j = 0; j < KK; ++j) s += cj[j] * (double)V[j * ldv + i];
     x[i] += s;
+  }
+}
+
+// f(std::integral_constant<int, k>) for a runtime 1 <= k <= kMaxM
+template <int LO, typename F>
+void with_count(int k, F&& f)
+{
+  if constexpr (LO <= kMaxM) {
+    if (k == LO) f(std::integral_constant<int, LO>{});
+    else with_count<LO + 1>(k, f);
   }
 }
 
@@ -287,9 +289,15 @@ int step(int64_t n, int m, int k, TV* V, int64_t ldv, void* wsp, double* resid_o
     return fail(GD_ERR_INVALID, "gd_arnoldi_step: bad arguments");
   Workspace* ws = static_cast<Workspace*>(wsp);
   const int nb = grid_for(n);
-  hipLaunchKernelGGL((k_pass_a<TV>), dim3(nb), dim3(kThreads), 0, st, n, k, V, ldv, ws);
-  hipLaunchKernelGGL((k_pass_b<TV>), dim3(nb), dim3(kThreads), 0, st, n, k, V, ldv, ws);
-  hipLaunchKernelGGL((k_pass_c<TV>), dim3(nb), dim3(kThreads), 0, st, n, k, V, ldv, ws, resid_out);
+  with_count<1>(k + 1, [&](auto KK_) {
+    constexpr int KK = decltype(KK_)::value;
+    hipLaunchKernelGGL((k_pass_a<TV, KK>), dim3(nb), dim3(kThreads), 0, st, n, V, ldv, ws);
+    hipLaunchKernelGGL(k_fin_a, dim3(1), dim3(kThreads), 0, st, nb, KK, ws);
+    hipLaunchKernelGGL((k_pass_b<TV, KK>), dim3(nb), dim3(kThreads), 0, st, n, V, ldv, ws);
+    hipLaunchKernelGGL(k_fin_b, dim3(1), dim3(kThreads), 0, st, nb, KK, ws);
+    hipLaunchKernelGGL((k_pass_c<TV, KK>), dim3(nb), dim3(kThreads), 0, st, n, V, ldv, ws);
+    hipLaunchKernelGGL(k_fin_c, dim3(1), dim3(kThreads), 0, st, nb, k, ws, resid_out);
+  });
   GD_HIP(hipGetLastError());
   return GD_OK;
 }
@@ -300,8 +308,11 @@ int update(int64_t n, int m, int k, const TV* V, int64_t ldv, const void* wsp, d
   if (n < 1 || m < 1 || m > kMaxM || k < 0 || k > m || !V || !wsp || !x || ldv < n)
     return fail(GD_ERR_INVALID, "gd_gmres_update: bad arguments");
   if (k == 0) return GD_OK;
-  hipLaunchKernelGGL((k_update_x<TV>), dim3(grid_for(n)), dim3(kThreads), 0, st, n, k, V, ldv,
-                     static_cast<const Workspace*>(wsp), x);
+  const Workspace* ws = static_cast<const Workspace*>(wsp);
+  with_count<1>(k, [&](auto KK_) {
+    constexpr int KK = decltype(KK_)::value;
+    hipLaunchKernelGGL((k_update_x<TV, KK>), dim3(grid_for(n)), dim3(kThreads), 0, st, n, V, ldv, ws, x);
+  });
   GD_HIP(hipGetLastError());
   return GD_OK;
 }
