@@ -8,6 +8,7 @@
 
 #include "gd_dispatch.hpp"
 #include "gd_k3d_api.hpp"
+#include "gd_k3k_api.hpp"
 #include "gd_padapt_api.hpp"
 
 namespace gd {
@@ -52,14 +53,18 @@ int check_adv(const gd_plan* plan, bool need_pointwise)
   return GD_OK;
 }
 
-// Lanes per element of the register-resident 3D kernel: from the tuning
-// table (autotuner), overridable with GENDIL_AMD_K3_LANES; 0 = generic kernel.
+// 3D kernel choice: from the tuning table (autotuner), overridable with
+// GENDIL_AMD_K3_LANES.  0 = generic kernel; 1, 2, 4 = register-resident kernel
+// with that many lanes per element; 16 + lanes = Kronecker-factored kernel.
+// Default: the Kronecker kernel where it applies, else the register kernel.
 template <typename T>
-int k3_lanes(const gd_plan* plan, int N)
+int k3_choice(const gd_plan* plan, int N, bool adv, bool mass)
 {
   if (const char* env = std::getenv("GENDIL_AMD_K3_LANES")) return std::atoi(env);
   const int tuned = tuned_lanes(plan, N, sizeof(T));
   if (tuned >= 0) return tuned;
+  const int kl = k3k::default_lanes(N, sizeof(T));
+  if (k3k::available<T>(plan, kl, adv, mass)) return 16 + kl;
   // measured on MI355X for the 64^3 p=3 workload: FP64 2 lanes, FP32 4 lanes
   if (N == 4) return sizeof(T) == 8 ? 2 : 4;
   return 1;
@@ -68,9 +73,10 @@ int k3_lanes(const gd_plan* plan, int N)
 // The register-resident kernel covers 3D GLL elements with q = p+1 and
 // one-axis separable (or constant) coefficients; see gd_k3d_host.cpp.
 template <typename T>
-bool k3_eligible(const gd_plan* plan, int lanes, bool adv, bool mass)
+bool k3_eligible(const gd_plan* plan, int choice, bool adv, bool mass)
 {
-  return k3::available<T>(plan, lanes, adv, mass);
+  if (choice >= 16) return k3k::available<T>(plan, choice - 16, adv, mass);
+  return k3::available<T>(plan, choice, adv, mass);
 }
 
 template <typename T>
@@ -108,7 +114,13 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
   const TableSet* ts = find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
   const Params<T> prm = make_params<T>(plan, ts, mass_coef, adv_coef);
   {
-    const int lanes = k3_lanes<T>(plan, plan->desc.order + 1);
+    const int N = plan->desc.order + 1;
+    const int choice = k3_choice<T>(plan, N, adv, mass_coef != T(0));
+    if (choice >= 16) {
+      const int r = k3k::apply<T>(plan, choice - 16, mass_coef, adv_coef, x, y, s);
+      if (r != GD_ERR_UNSUPPORTED) return r;
+    }
+    const int lanes = choice >= 16 ? (N == 4 ? (sizeof(T) == 8 ? 2 : 4) : 1) : choice;
     if (lanes > 0) {
       const int r = k3::apply<T>(plan, lanes, mass_coef, adv_coef, x, y, s);
       if (r != GD_ERR_UNSUPPORTED) return r;
@@ -198,7 +210,7 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   if (plan->padapt)
     for (int v = 0; v < gd::pa::num_variants(plan->dim); ++v) choices.push_back(v);
   else
-    choices = {0, 1, 2, 4};
+    choices = {0, 1, 2, 4, 17, 18, 20};
   const int saved = *knob;
   double best = 1e300;
   int best_lanes = 0;
@@ -245,6 +257,30 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   return GD_OK;
 }
 
+int gd_plan_query_variant(const gd_plan* plan, int32_t precision_bytes, double mass_coef, double adv_coef,
+                          int32_t* variant)
+{
+  if (!plan || !variant || (precision_bytes != 4 && precision_bytes != 8))
+    return fail(GD_ERR_INVALID, "gd_plan_query_variant: bad arguments");
+  if (plan->padapt) {
+    const int v = plan->tuned_pa[precision_bytes == 8 ? 0 : 1];
+    *variant = v < 0 ? 0 : v;
+    return GD_OK;
+  }
+  const bool adv = adv_coef != 0.0, mass = mass_coef != 0.0;
+  const int N = plan->desc.order + 1;
+  auto resolve = [&](auto tag) -> int {
+    using T = decltype(tag);
+    const int choice = gd::k3_choice<T>(plan, N, adv, mass);
+    if (choice >= 16 && gd::k3k::available<T>(plan, choice - 16, adv, mass)) return choice;
+    const int lanes = choice >= 16 ? (N == 4 ? (sizeof(T) == 8 ? 2 : 4) : 1) : choice;
+    if (lanes > 0 && gd::k3::available<T>(plan, lanes, adv, mass)) return lanes;
+    return 0;
+  };
+  *variant = precision_bytes == 8 ? resolve(0.0) : resolve(0.0f);
+  return GD_OK;
+}
+
 int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes)
 {
   if (!plan || (precision_bytes != 4 && precision_bytes != 8)) return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
@@ -255,7 +291,8 @@ int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes)
     plan->tuned_pa[slot] = lanes;
     return GD_OK;
   }
-  if (lanes != -1 && lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4)
+  if (lanes != -1 && lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 17 && lanes != 18 &&
+      lanes != 20)
     return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
   plan->tuned_k3_lanes[slot] = lanes;
   return GD_OK;

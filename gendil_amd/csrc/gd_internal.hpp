@@ -98,6 +98,10 @@ namespace pa {
 struct State;
 void destroy_state(State* s);
 }  // namespace pa
+namespace k3k {
+struct State;
+void destroy_state(State* s);
+}  // namespace k3k
 
 }  // namespace gd
 
@@ -130,14 +134,17 @@ struct gd_plan {
   std::vector<gd::FieldDev<double>> fields64_host;  // host copies of the device descriptors
   std::vector<gd::FieldDev<float>> fields32_host;
   std::vector<void*> field_tables;  // device table allocations
-  // autotuned lanes per element of the register-resident 3D kernel
-  // ([0] FP64, [1] FP32; -1 = not tuned, 0 = generic kernel)
+  // autotuned 3D kernel choice ([0] FP64, [1] FP32): -1 = not tuned,
+  // 0 = generic kernel, 1/2/4 = register kernel with that many lanes per
+  // element, 16 + lanes = Kronecker-factored kernel
   int tuned_k3_lanes[2] = {-1, -1};
   // p-adaptive path: kernel variant per precision (-1 = default), lazily
   // built element buckets and device parameter block, and a counter bumped by
   // every coefficient change so the device copy is refreshed
   int tuned_pa[2] = {-1, -1};
   mutable gd::pa::State* pa_state = nullptr;
+  // factor tables of the Kronecker-factored 3D kernel (lazily built)
+  mutable gd::k3k::State* k3k_state = nullptr;
   int coef_version = 0;
   ~gd_plan();
 };

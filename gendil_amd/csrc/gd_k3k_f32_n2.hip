@@ -1,0 +1,12 @@
+// Kronecker-factored 3D kernel instances: float, 2 nodes per dimension.
+#include "gd_k3k_inst.hpp"
+
+namespace gd {
+namespace k3k {
+
+static const KEntry<float> kList[] = {GD_K3K_FOR_LANES(float, 2, 1), GD_K3K_FOR_LANES(float, 2, 2)};
+
+KList<float> instances_f32_n2() { return KList<float>{kList, (int)(sizeof(kList) / sizeof(kList[0]))}; }
+
+}  // namespace k3k
+}  // namespace gd

@@ -1,0 +1,12 @@
+// Kronecker-factored 3D kernel instances: float, 3 nodes per dimension.
+#include "gd_k3k_inst.hpp"
+
+namespace gd {
+namespace k3k {
+
+static const KEntry<float> kList[] = {GD_K3K_FOR_LANES(float, 3, 1)};
+
+KList<float> instances_f32_n3() { return KList<float>{kList, (int)(sizeof(kList) / sizeof(kList[0]))}; }
+
+}  // namespace k3k
+}  // namespace gd

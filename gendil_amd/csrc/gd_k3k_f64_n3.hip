@@ -1,0 +1,12 @@
+// Kronecker-factored 3D kernel instances: double, 3 nodes per dimension.
+#include "gd_k3k_inst.hpp"
+
+namespace gd {
+namespace k3k {
+
+static const KEntry<double> kList[] = {GD_K3K_FOR_LANES(double, 3, 1)};
+
+KList<double> instances_f64_n3() { return KList<double>{kList, (int)(sizeof(kList) / sizeof(kList[0]))}; }
+
+}  // namespace k3k
+}  // namespace gd

@@ -345,6 +345,7 @@ gd_plan::~gd_plan()
   if (elem_quad_dev) (void)hipFree(elem_quad_dev);
   gd::free_fields(this);
   gd::pa::destroy_state(pa_state);
+  gd::k3k::destroy_state(k3k_state);
 }
 
 using gd::fail;

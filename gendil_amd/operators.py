@@ -122,9 +122,19 @@ class Plan:
         return chosen.value, best.value
 
     def set_variant(self, dtype, lanes: int):
-        """Pin the kernel variant: -1 default, 0 generic, 1/2/4 lanes per element."""
+        """Pin the kernel variant: -1 default, 0 generic, 1/2/4 lanes per element
+        (register kernel), 17/18/20 (Kronecker kernel, 16 + lanes); p-adaptive
+        plans: the layout index."""
         nbytes = 8 if dtype == torch.float64 else 4
         L.check(L.lib().gd_plan_set_variant(self.handle, nbytes, lanes), "gd_plan_set_variant")
+
+    def query_variant(self, dtype, mass_coef: float, adv_coef: float) -> int:
+        """The kernel variant an apply with these coefficients would run."""
+        nbytes = 8 if dtype == torch.float64 else 4
+        v = C.c_int32()
+        L.check(L.lib().gd_plan_query_variant(self.handle, nbytes, mass_coef, adv_coef, C.byref(v)),
+                "gd_plan_query_variant")
+        return v.value
 
     # ---- applies ----
     def apply(self, mass_coef: float, adv_coef: float, x: torch.Tensor, y: torch.Tensor):

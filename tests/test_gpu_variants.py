@@ -1,6 +1,8 @@
 """Every kernel variant (generic LDS-staged kernel, register-resident 3D
-kernel with 1, 2 or 4 lanes per element) against GenDiL's golden vectors,
-and the autotuner's choice."""
+kernel with 1, 2 or 4 lanes per element, Kronecker-factored 3D kernel with
+16 + lanes) against GenDiL's golden vectors and the oracle, and the
+autotuner's choice.  A pinned variant that does not serve a plan falls back
+to one that does, so every case must still be correct."""
 import numpy as np
 import pytest
 import torch
@@ -13,11 +15,14 @@ from gendil_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 ELIGIBLE = ["adv3d_c2_p3q4_per", "adv3d_c2_p3q4_bc0", "adv3d_c2_p3q4_nobc", "adv3d_trig_p3q4_bcpoly",
-            "mass3d_p3q4_one", "adv3d_trig_p2q3_m1per"]
+            "mass3d_p3q4_one", "adv3d_trig_p2q3_m1per", "adv3d_const_p2q4_per", "adv3d_bubble_p1q3",
+            "adv3d_bubble_p2q4", "adv3d_bubble_p3q5"]
+KRON = {1: [17, 18], 2: [17], 3: [18, 20]}
 
 
 def variants(case):
-    return [0, 1, 2, 4] if case.P == 3 else [0, 1]
+    base = [0, 1, 2, 4] if case.P == 3 else [0, 1]
+    return base + KRON.get(case.P, [])
 
 
 @pytest.mark.parametrize("name", ELIGIBLE)
@@ -36,7 +41,7 @@ def test_all_variants_match_golden(name, dtype):
         assert err <= (1e-12 if dtype == torch.float64 else 1e-5), (lanes, err)
 
 
-@pytest.mark.parametrize("lanes", [0, 1, 2, 4])
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 18, 20])
 def test_variants_combined_operator_at_16_cubed(lanes):
     mesh = G.CartesianMesh.unit(3, 16, False)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
@@ -65,3 +70,53 @@ def test_autotune_picks_a_correct_variant():
     op(torch.as_tensor(x, device="cuda"), y)
     ref = oracle.Problem(fes, ir, adv_coef=1.0, adv=W.advection_c2()).apply(x)
     assert rel_l2(y.cpu().numpy(), ref) <= 1e-12
+
+
+def test_kronecker_kernel_is_the_default_for_the_benchmark_field():
+    mesh = G.CartesianMesh.unit(3, 8, True)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    op = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(4), W.advection_c2())
+    assert op.plan.query_variant(torch.float64, 0.0, 1.0) == 20
+    assert op.plan.query_variant(torch.float32, 0.0, 1.0) == 20
+    # a field whose sign changes along a face is served by the register kernel
+    bad = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(4), beta_field("trig", 3))
+    assert bad.plan.query_variant(torch.float64, 0.0, 1.0) in (1, 2, 4)
+
+
+KRON_CASES = [
+    # (p, n, periodic, bc, beta kind, mass_coef, adv_coef)
+    (1, 5, True, None, "c2", 0.0, 1.0),
+    (2, 4, False, 0.0, "c2", 0.0, 1.0),
+    (2, 4, False, None, "c2", 1.0, 0.3),
+    (3, 3, False, 0.0, "const", 0.0, 1.0),
+    (3, 4, True, None, "negc2", 0.0, 1.0),
+    (3, 3, False, None, "negc2", 0.7, 1.0),
+]
+
+
+def negc2():
+    # the benchmark field with every component negated (upwind sides swap)
+    from gendil_amd.fields import SeparableField, VectorField
+    comps = W.advection_c2().components
+    return VectorField([SeparableField([(-c, f) for c, f in comp.terms]) for comp in comps])
+
+
+@pytest.mark.parametrize("case", KRON_CASES, ids=[f"p{c[0]}_n{c[1]}_{c[4]}_bc{c[3]}_m{c[5]}" for c in KRON_CASES])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_kronecker_kernel_vs_oracle(case, dtype):
+    p, n, periodic, bc, kind, mc, ac = case
+    mesh = G.CartesianMesh.unit(3, n, periodic)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(p))
+    ir = G.MakeIntegrationRule(p + 1)
+    adv = negc2() if kind == "negc2" else beta_field(kind, 3)
+    op = G.MakeMassAdvectionOperator(fes, ir, adv, bc=bc, mass_coef=mc, adv_coef=ac)
+    chosen = op.plan.query_variant(dtype, mc, ac)
+    print(f"kronecker case {case} {dtype}: variant {chosen}", flush=True)
+    assert chosen >= 16
+    x = G.fill_random(op.num_dofs, 9)
+    y = torch.empty(op.num_dofs, dtype=dtype, device="cuda")
+    op(torch.as_tensor(x, dtype=dtype, device="cuda"), y)
+    ref = oracle.Problem(fes, ir, mass_coef=mc, adv_coef=ac, adv=adv, bc=bc).apply(x)
+    err = rel_l2(y.double().cpu().numpy(), ref)
+    print(f"rel l2 = {err:.3e}")
+    assert err <= (1e-12 if dtype == torch.float64 else 1e-5)
