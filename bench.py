@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES comparison")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-padapt", action="store_true", help="skip the 6D p-adaptive apply")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also time a plain device copy of the apply's input/output bytes (counter calibration)")
     return ap.parse_args()
 
 
@@ -239,6 +241,14 @@ def main():
             "traffic": None,
         },
     }
+    if args.copy_calibration:
+        # a plain copy moving the apply's algorithmic bytes (read N, write N
+        # values): hardware byte counters of the apply are scaled by what the
+        # same counters read for this kernel
+        for i in range(args.steps):
+            ys[i % 3].copy_(xs[i % 3])
+        torch.cuda.synchronize()
+        result["copy_calibration_bytes"] = bytes_per_apply
     del ys
     if not args.no_fp32:
         x32 = [x.float() for x in xs]

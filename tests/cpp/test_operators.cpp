@@ -1,0 +1,188 @@
+// C++20 host API test: the header-only operators (include/gendil_amd/
+// operators.hpp) against the oracle (liboracle.so, test infrastructure), and
+// the Krylov solvers.  Exit status 0 = every check passed.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "gd_oracle.h"
+#include "gendil_amd/operators.hpp"
+
+namespace ga = gendil_amd;
+
+static int failures = 0;
+
+static void expect(bool ok, const char* what, double value)
+{
+  std::printf("%-58s %.3e  %s\n", what, value, ok ? "ok" : "FAIL");
+  if (!ok) ++failures;
+}
+
+static double rel_l2(const std::vector<double>& a, const std::vector<double>& b)
+{
+  double num = 0.0, den = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    num += (a[i] - b[i]) * (a[i] - b[i]);
+    den += b[i] * b[i];
+  }
+  return std::sqrt(num / den);
+}
+
+constexpr double kTwoPi = 6.283185307179586;
+
+// the benchmark field: (1 + 0.5 sin 2 pi y, 0.5 + 0.5 cos 2 pi z, 0.25 + 0.25 sin 2 pi x)
+static double bx(double y) { return 1.0 + 0.5 * std::sin(kTwoPi * y); }
+static double by(double z) { return 0.5 + 0.5 * std::cos(kTwoPi * z); }
+static double bz(double x) { return 0.25 + 0.25 * std::sin(kTwoPi * x); }
+
+static ga::VectorField<3> c2()
+{
+  return ga::VectorField<3>{{ga::SeparableField::OneAxis(1, bx), ga::SeparableField::OneAxis(2, by),
+                             ga::SeparableField::OneAxis(0, bz)}};
+}
+
+// oracle problem for a uniform 3D space with the benchmark field
+struct OracleC2 {
+  gdo_problem p{};
+  std::vector<double> data;
+  OracleC2(const ga::CartesianMesh<3>& m, int order, int nq, double mass_coef, double adv_coef, int bc_mode)
+  {
+    p.dim = 3;
+    for (int d = 0; d < 3; ++d) {
+      p.sizes[d] = m.sizes[d];
+      p.h[d] = m.h[d];
+      p.origin[d] = m.origin[d];
+      p.periodic[d] = m.periodic[d];
+    }
+    p.order = order;
+    p.nq = nq;
+    p.basis = GDO_BASIS_GLL;
+    p.mass_coef = mass_coef;
+    p.adv_coef = adv_coef;
+    p.sigma.rank = 1;
+    p.sigma.scale[0] = 1.0;
+    for (int d = 0; d < GDO_MAX_DIM; ++d) p.sigma.offset[0][d] = -1;
+    p.beta_kind = GDO_BETA_SEPARABLE;
+    double (*fn[3])(double) = {bx, by, bz};
+    const int axis[3] = {1, 2, 0};
+    std::vector<int64_t> off(3);
+    for (int c = 0; c < 3; ++c) {
+      std::vector<double> pts((size_t)m.sizes[axis[c]] * (nq + 2));
+      gdo_axis_points(&p, axis[c], pts.data());
+      off[c] = (int64_t)data.size();
+      for (double x : pts) data.push_back(fn[c](x));
+    }
+    for (int c = 0; c < 3; ++c) {
+      p.beta[c].rank = 1;
+      p.beta[c].scale[0] = 1.0;
+      for (int d = 0; d < GDO_MAX_DIM; ++d) p.beta[c].offset[0][d] = -1;
+      p.beta[c].offset[0][axis[c]] = off[c];
+      p.beta[c].data = data.data();
+    }
+    p.bc_mode = bc_mode;
+    p.bc.rank = 0;
+  }
+  std::vector<double> apply(const std::vector<double>& x)
+  {
+    std::vector<double> y(x.size());
+    gdo_apply(&p, x.data(), y.data());
+    return y;
+  }
+};
+
+int main()
+{
+  try {
+    // ---- advection apply, 6^3 periodic, p = 3, q = 4 ----
+    {
+      auto mesh = ga::MakeUnitCartesianMesh<3>(6, true);
+      auto fes = ga::MakeFiniteElementSpace(mesh, ga::MakeLobattoFiniteElement(3));
+      auto ir = ga::MakeIntegrationRule(4);
+      auto A = ga::MakeAdvectionOperator(fes, ir, c2());
+      const int64_t n = A.num_dofs();
+      const std::vector<double> xh = ga::FillRandomHost(n, 12345);
+      ga::Vector<double> x(n), y(n);
+      x.FromHost(xh);
+      A(x, y);
+      OracleC2 ref(mesh, 3, 4, 0.0, 1.0, GDO_BC_NONE);
+      const double e64 = rel_l2(y.ToHost(), ref.apply(xh));
+      expect(e64 <= 1e-12, "advection 6^3 p=3 FP64 vs oracle (rel l2)", e64);
+      std::vector<float> xf(xh.begin(), xh.end());
+      ga::Vector<float> x32(n), y32(n);
+      x32.FromHost(xf);
+      A(x32, y32);
+      const std::vector<float> yf = y32.ToHost();
+      const double e32 = rel_l2(std::vector<double>(yf.begin(), yf.end()), ref.apply(xh));
+      expect(e32 <= 1e-5, "advection 6^3 p=3 FP32 vs oracle (rel l2)", e32);
+    }
+    // ---- CG on the mass matrix ----
+    {
+      auto mesh = ga::MakeUnitCartesianMesh<3>(5, false);
+      auto fes = ga::MakeFiniteElementSpace(mesh, ga::MakeLobattoFiniteElement(2));
+      auto M = ga::MakeMassFiniteElementOperator(fes, ga::MakeIntegrationRule(3));
+      const int64_t n = M.num_dofs();
+      const std::vector<double> xt = ga::FillRandomHost(n, 7);
+      ga::Vector<double> xtv(n), b(n), x(n);
+      xtv.FromHost(xt);
+      M(xtv, b);
+      x.FromHost(std::vector<double>((size_t)n, 0.0));
+      ga::SolveOptions opt;
+      opt.rel_tol = 1e-12;
+      const ga::SolveReport rep = ga::ConjugateGradient(M, b, x, opt);
+      const double e = rel_l2(x.ToHost(), xt);
+      expect(rep.success && e <= 1e-9, "CG on M, 5^3 p=2: solution error", e);
+    }
+    // ---- GMRES (FP64 and mixed) on M + dt A with inflow g = 0 ----
+    {
+      auto mesh = ga::MakeUnitCartesianMesh<3>(4, false);
+      auto fes = ga::MakeFiniteElementSpace(mesh, ga::MakeLobattoFiniteElement(3));
+      auto ir = ga::MakeIntegrationRule(4);
+      const double dt = 0.5 * 0.25 / std::sqrt(3.5);
+      auto plan = std::make_shared<ga::Plan>(fes, ir);
+      plan->set_sigma(ga::SeparableField::Constant(1.0));
+      plan->set_beta(c2());
+      plan->set_inflow(ga::SeparableField::Constant(0.0));
+      ga::Operator K(plan, 1.0, dt);
+      const int64_t n = K.num_dofs();
+      const std::vector<double> bh = ga::FillRandomHost(n, 3);
+      ga::Vector<double> b(n), x(n);
+      b.FromHost(bh);
+      OracleC2 ref(mesh, 3, 4, 1.0, dt, GDO_BC_INFLOW);
+      for (int mixed = 0; mixed < 2; ++mixed) {
+        x.FromHost(std::vector<double>((size_t)n, 0.0));
+        ga::SolveOptions opt;
+        opt.rel_tol = 1e-10;
+        const ga::SolveReport rep = mixed ? ga::GMRESMixed(K, b, x, opt) : ga::GMRES(K, b, x, opt);
+        const std::vector<double> kx = ref.apply(x.ToHost());
+        double num = 0.0, den = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+          num += (bh[i] - kx[i]) * (bh[i] - kx[i]);
+          den += bh[i] * bh[i];
+        }
+        const double r = std::sqrt(num / den);
+        expect(rep.success && r <= 2e-10,
+               mixed ? "GMRES mixed (M + dt A) 4^3: oracle residual" : "GMRES FP64 (M + dt A) 4^3: oracle residual",
+               r);
+      }
+    }
+    // ---- errors cross the ABI as exceptions ----
+    {
+      bool threw = false;
+      try {
+        auto mesh = ga::MakeUnitCartesianMesh<3>(2, true);
+        auto fes = ga::MakeFiniteElementSpace(mesh, ga::MakeLobattoFiniteElement(3));
+        auto A = ga::MakeAdvectionOperator(fes, ga::MakeIntegrationRule(4), c2());
+        ga::Vector<double> v(A.num_dofs());
+        A(v.ReadDeviceData(), v.WriteDeviceData());  // aliasing is rejected
+      } catch (const ga::Error& e) {
+        threw = e.code == GD_ERR_INVALID;
+      }
+      expect(threw, "aliased apply raises gendil_amd::Error(GD_ERR_INVALID)", threw ? 0.0 : 1.0);
+    }
+  } catch (const std::exception& e) {
+    std::printf("exception: %s\n", e.what());
+    return 2;
+  }
+  std::printf("%d failure(s)\n", failures);
+  return failures ? 1 : 0;
+}
