@@ -33,6 +33,7 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kMaxM = 32;          // restart length
 constexpr int kMaxBlocks = 1024;
 constexpr int kStride = kMaxM + 2; // partial columns per block: k+1 dots + norm
+constexpr int kFinThreads = 1024;  // finalize kernels: one wave per partial column
 
 struct Workspace {
   double H[kMaxM][kMaxM + 1];  // column k, rows 0..k+1 (rotated in place)
@@ -42,7 +43,7 @@ struct Workspace {
   double c[kMaxM];   // coefficients on the stored vectors for the next update pass
   double h1[kMaxM];  // accumulated true GS coefficients of the current column
   double pnorm, pinv;
-  double partial[kMaxBlocks][kStride];
+  double partial[kStride][kMaxBlocks];  // [column][block]: a column is contiguous
 };
 
 inline int grid_for(int64_t n)
@@ -58,9 +59,9 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
-// Block sums of the first `cols` accumulators into partial[blockIdx.x][:].
+// Block sums of the first `cols` accumulators into partial[:][blockIdx.x].
 template <int K>
-__device__ __forceinline__ void store_block_partials(const double (&acc)[K], int cols, double* out)
+__device__ __forceinline__ void store_block_partials(const double (&acc)[K], int cols, Workspace* ws)
 {
   __shared__ double red[kWaves][K];
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
@@ -76,21 +77,29 @@ __device__ __forceinline__ void store_block_partials(const double (&acc)[K], int
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) s += red[w][j];
-    out[j] = s;
+    ws->partial[j][blockIdx.x] = s;
   }
 }
 
-// out[j] = sum over blocks of partial[b][j], j < cols: one wave per column,
-// fixed order.  Results land in shared memory `col` (visible to all threads
-// after the trailing barrier).
+// out[j] = sum over blocks of partial[j][b], j < cols: one wave per column
+// (contiguous, all loads in flight together), fixed order.  Results land in
+// shared memory `col` (visible to all threads after the trailing barrier).
 __device__ __forceinline__ void sum_columns(const Workspace* ws, int nblk, int cols, double* col)
 {
   const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-  for (int j = wv; j < cols; j += kWaves) {
-    double v = 0.0;
-    for (int b = lane; b < nblk; b += 64) v += ws->partial[b][j];
-    v = wave_sum(v);
-    if (lane == 0) col[j] = v;
+  for (int j = wv; j < cols; j += kFinThreads / 64) {
+    const double* c = ws->partial[j];
+    double v[kMaxBlocks / 64];
+#pragma unroll
+    for (int i = 0; i < kMaxBlocks / 64; ++i) {
+      const int b = lane + 64 * i;
+      v[i] = b < nblk ? c[b] : 0.0;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < kMaxBlocks / 64; ++i) s += v[i];
+    s = wave_sum(s);
+    if (lane == 0) col[j] = s;
   }
   __syncthreads();
 }
@@ -115,14 +124,14 @@ __global__ void __launch_bounds__(kThreads) k_pass_a(int64_t n, const TV* __rest
 #pragma unroll
     for (int j = 0; j < KK; ++j) acc[j] += (double)V[j * ldv + i] * wi;
   }
-  store_block_partials<KK>(acc, KK, ws->partial[blockIdx.x]);
+  store_block_partials<KK>(acc, KK, ws);
 }
 
-__global__ void __launch_bounds__(kThreads) k_fin_a(int nblk, int kk, Workspace* __restrict__ ws)
+__global__ void __launch_bounds__(kFinThreads) k_fin_a(int nblk, int kk, Workspace* __restrict__ ws)
 {
   __shared__ double col[kStride];
   sum_columns(ws, nblk, kk, col);
-  for (int j = threadIdx.x; j < kk; j += kThreads) {
+  for (int j = threadIdx.x; j < kk; j += kFinThreads) {
     const double h = ws->s[j] * col[j];
     ws->h1[j] = h;
     ws->c[j] = ws->s[j] * h;
@@ -156,10 +165,10 @@ __global__ void __launch_bounds__(kThreads) k_pass_b(int64_t n, TV* __restrict__
     for (int j = 0; j < KK; ++j) acc[j] += v[j] * wr;
     acc[KK] += wr * wr;
   }
-  store_block_partials<KK + 1>(acc, KK + 1, ws->partial[blockIdx.x]);
+  store_block_partials<KK + 1>(acc, KK + 1, ws);
 }
 
-__global__ void __launch_bounds__(kThreads) k_fin_b(int nblk, int kk, Workspace* __restrict__ ws)
+__global__ void __launch_bounds__(kFinThreads) k_fin_b(int nblk, int kk, Workspace* __restrict__ ws)
 {
   __shared__ double col[kStride];
   sum_columns(ws, nblk, kk + 1, col);
@@ -198,11 +207,11 @@ __global__ void __launch_bounds__(kThreads) k_pass_c(int64_t n, TV* __restrict__
     const double zr = (double)zt;
     acc[0] += zr * zr;
   }
-  store_block_partials<1>(acc, 1, ws->partial[blockIdx.x]);
+  store_block_partials<1>(acc, 1, ws);
 }
 
 // Finishes column k of the Hessenberg matrix and its Givens rotation.
-__global__ void __launch_bounds__(kThreads) k_fin_c(int nblk, int k, Workspace* __restrict__ ws,
+__global__ void __launch_bounds__(kFinThreads) k_fin_c(int nblk, int k, Workspace* __restrict__ ws,
                                                     double* __restrict__ resid_out)
 {
   __shared__ double col[kStride];
@@ -292,11 +301,11 @@ int step(int64_t n, int m, int k, TV* V, int64_t ldv, void* wsp, double* resid_o
   with_count<1>(k + 1, [&](auto KK_) {
     constexpr int KK = decltype(KK_)::value;
     hipLaunchKernelGGL((k_pass_a<TV, KK>), dim3(nb), dim3(kThreads), 0, st, n, V, ldv, ws);
-    hipLaunchKernelGGL(k_fin_a, dim3(1), dim3(kThreads), 0, st, nb, KK, ws);
+    hipLaunchKernelGGL(k_fin_a, dim3(1), dim3(kFinThreads), 0, st, nb, KK, ws);
     hipLaunchKernelGGL((k_pass_b<TV, KK>), dim3(nb), dim3(kThreads), 0, st, n, V, ldv, ws);
-    hipLaunchKernelGGL(k_fin_b, dim3(1), dim3(kThreads), 0, st, nb, KK, ws);
+    hipLaunchKernelGGL(k_fin_b, dim3(1), dim3(kFinThreads), 0, st, nb, KK, ws);
     hipLaunchKernelGGL((k_pass_c<TV, KK>), dim3(nb), dim3(kThreads), 0, st, n, V, ldv, ws);
-    hipLaunchKernelGGL(k_fin_c, dim3(1), dim3(kThreads), 0, st, nb, k, ws, resid_out);
+    hipLaunchKernelGGL(k_fin_c, dim3(1), dim3(kFinThreads), 0, st, nb, k, ws, resid_out);
   });
   GD_HIP(hipGetLastError());
   return GD_OK;

@@ -64,7 +64,7 @@ def test_autotune_picks_a_correct_variant():
     for dtype in (torch.float64, torch.float32):
         lanes, ms = op.plan.autotune(dtype, 0.0, 1.0, reps=3)
         print(f"autotune {dtype}: lanes={lanes} {ms:.4f} ms", flush=True)
-        assert lanes in (0, 1, 2, 4) and ms > 0
+        assert lanes in (0, 1, 2, 4, 17, 18, 20) and ms > 0
     x = G.fill_random(op.num_dofs, 1)
     y = torch.empty(op.num_dofs, dtype=torch.float64, device="cuda")
     op(torch.as_tensor(x, device="cuda"), y)
