@@ -55,14 +55,19 @@ int check_adv(const gd_plan* plan, bool need_pointwise)
 
 // 3D kernel choice: from the tuning table (autotuner), overridable with
 // GENDIL_AMD_K3_LANES.  0 = generic kernel; 1, 2, 4 = register-resident kernel
-// with that many lanes per element; 16 + lanes = Kronecker-factored kernel.
-// Default: the Kronecker kernel where it applies, else the register kernel.
+// with that many lanes per element; 16 + lanes = Kronecker-factored kernel
+// (lanes 1, 2, 4: register-resident; 32..39: LDS-staged, bit 0 = outputs
+// staged through LDS, bit 1 = 128-thread blocks, bit 2 = one term at a time).  Default: the staged
+// Kronecker kernel where it applies, else the register Kronecker kernel, else
+// the register kernel.
 template <typename T>
 int k3_choice(const gd_plan* plan, int N, bool adv, bool mass)
 {
   if (const char* env = std::getenv("GENDIL_AMD_K3_LANES")) return std::atoi(env);
   const int tuned = tuned_lanes(plan, N, sizeof(T));
   if (tuned >= 0) return tuned;
+  const int ks = k3k::default_staged(N, sizeof(T));
+  if (ks >= 0 && k3k::available<T>(plan, ks, adv, mass)) return 16 + ks;
   const int kl = k3k::default_lanes(N, sizeof(T));
   if (k3k::available<T>(plan, kl, adv, mass)) return 16 + kl;
   // measured on MI355X for the 64^3 p=3 workload: FP64 2 lanes, FP32 4 lanes
@@ -117,8 +122,12 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
     const int N = plan->desc.order + 1;
     const int choice = k3_choice<T>(plan, N, adv, mass_coef != T(0));
     if (choice >= 16) {
-      const int r = k3k::apply<T>(plan, choice - 16, mass_coef, adv_coef, x, y, s);
+      int r = k3k::apply<T>(plan, choice - 16, mass_coef, adv_coef, x, y, s);
       if (r != GD_ERR_UNSUPPORTED) return r;
+      if (choice >= 48) {  // staged variant refused (e.g. unaligned vectors): register variant
+        r = k3k::apply<T>(plan, k3k::default_lanes(N, sizeof(T)), mass_coef, adv_coef, x, y, s);
+        if (r != GD_ERR_UNSUPPORTED) return r;
+      }
     }
     const int lanes = choice >= 16 ? (N == 4 ? (sizeof(T) == 8 ? 2 : 4) : 1) : choice;
     if (lanes > 0) {
@@ -210,7 +219,7 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   if (plan->padapt)
     for (int v = 0; v < gd::pa::num_variants(plan->dim); ++v) choices.push_back(v);
   else
-    choices = {0, 1, 2, 4, 17, 18, 20};
+    choices = {0, 1, 2, 4, 17, 18, 20, 48, 49, 50, 51, 52, 53, 54, 55};
   const int saved = *knob;
   double best = 1e300;
   int best_lanes = 0;
@@ -292,7 +301,7 @@ int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes)
     return GD_OK;
   }
   if (lanes != -1 && lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 17 && lanes != 18 &&
-      lanes != 20)
+      lanes != 20 && !(lanes >= 48 && lanes <= 55))
     return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
   plan->tuned_k3_lanes[slot] = lanes;
   return GD_OK;

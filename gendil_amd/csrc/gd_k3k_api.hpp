@@ -46,8 +46,11 @@ int apply(const gd_plan* plan, int lanes, T mass_coef, T adv_coef, const T* x, T
 template <typename T>
 bool available(const gd_plan* plan, int lanes, bool adv, bool mass);
 
-// Lane counts compiled for n nodes per dimension, best first.
+// Lane count of the register-resident Kronecker variant for n nodes per
+// dimension, and the code (32..39) of the default LDS-staged variant (-1: none
+// compiled for n).
 int default_lanes(int n, size_t elem_bytes);
+int default_staged(int n, size_t elem_bytes);
 
 }  // namespace k3k
 }  // namespace gd

@@ -4,7 +4,7 @@
 namespace gd {
 namespace k3k {
 
-static const KEntry<float> kList[] = {GD_K3K_FOR_LANES(float, 2, 1), GD_K3K_FOR_LANES(float, 2, 2)};
+static const KEntry<float> kList[] = {GD_K3K_FOR_LANES(float, 2, 1), GD_K3K_FOR_LANES(float, 2, 2), GD_K3K_STAGED_ALL(float, 2)};
 
 KList<float> instances_f32_n2() { return KList<float>{kList, (int)(sizeof(kList) / sizeof(kList[0]))}; }
 

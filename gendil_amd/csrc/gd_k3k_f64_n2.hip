@@ -4,7 +4,7 @@
 namespace gd {
 namespace k3k {
 
-static const KEntry<double> kList[] = {GD_K3K_FOR_LANES(double, 2, 1), GD_K3K_FOR_LANES(double, 2, 2)};
+static const KEntry<double> kList[] = {GD_K3K_FOR_LANES(double, 2, 1), GD_K3K_FOR_LANES(double, 2, 2), GD_K3K_STAGED_ALL(double, 2)};
 
 KList<double> instances_f64_n2() { return KList<double>{kList, (int)(sizeof(kList) / sizeof(kList[0]))}; }
 

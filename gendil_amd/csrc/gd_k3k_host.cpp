@@ -2,6 +2,7 @@
 // factor tables, kernel-argument packing and instance lookup.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <vector>
 
 #include "gd_k3k.hpp"
@@ -38,6 +39,14 @@ int default_lanes(int n, size_t elem_bytes)
   if (n == 4) return 4;
   if (n == 2) return 2;
   return 1;
+}
+
+int default_staged(int n, size_t elem_bytes)
+{
+  if (n != 4 && n != 2) return -1;
+  // measured on MI355X at 64^3, p = 3: one-pass z stage, outputs staged
+  // through LDS; FP64 in 256-thread blocks, FP32 in 128-thread blocks
+  return elem_bytes == 8 ? 32 + 1 : 32 + 2 + 1;
 }
 
 namespace {
@@ -222,6 +231,9 @@ int apply(const gd_plan* plan, int lanes, T mass_coef, T adv_coef, const T* x, T
   KLaunch<T> fn = nullptr;
   const bool adv = adv_coef != T(0), mass = mass_coef != T(0);
   if (!prepare<T>(plan, lanes, adv, mass, prm, &fn)) return GD_ERR_UNSUPPORTED;
+  // the LDS-staged variants move 16-byte chunks
+  if (lanes >= 32 && ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15) != 0)
+    return GD_ERR_UNSUPPORTED;
   for (int d = 0; d < 3; ++d) prm.Kt[d] *= adv_coef;
   prm.Km *= mass_coef;
   return fn(prm, x, y, s);

@@ -22,6 +22,23 @@ int launch(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
   return GD_OK;
 }
 
+// LDS-staged kernel: lanes code 32 + (1: outputs staged through LDS) + (2:
+// blocks of 128 threads instead of 256) + (4: one term at a time)
+template <typename T, int N, int CODE, bool ADV, bool MASS, int SIG>
+int launch_staged(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
+{
+  constexpr int NT = (CODE & 2) ? 128 : 256;
+  constexpr int EPB = NT / N;
+  KParams<T> prm = prm_in;
+  const long long nb = ((long long)prm.E + EPB - 1) / EPB;
+  prm.num_blocks = (int)nb;
+  if (nb == 0) return GD_OK;
+  hipLaunchKernelGGL((k3k_staged<T, N, ADV, MASS, SIG, EPB, (CODE & 1) != 0, (CODE & 4) != 0>), dim3((unsigned)nb),
+                     dim3(NT), 0, s, prm, x, y);
+  GD_HIP(hipGetLastError());
+  return GD_OK;
+}
+
 // signatures compiled: the benchmark field (beta_x(y), beta_y(z), beta_z(x))
 // and constant fields
 constexpr int kSigCyclic = ksig(1, 2, 0);
@@ -33,6 +50,16 @@ constexpr int kSigConst = ksig(-1, -1, -1);
   GD_K3K_ENTRY(T, N, L, 1, 0, kSigCyclic), GD_K3K_ENTRY(T, N, L, 1, 1, kSigCyclic),                         \
       GD_K3K_ENTRY(T, N, L, 1, 0, kSigConst), GD_K3K_ENTRY(T, N, L, 1, 1, kSigConst),                       \
       GD_K3K_ENTRY(T, N, L, 0, 1, kSigConst)
+
+#define GD_K3K_STAGED_ENTRY(T, N, CODE, ADV, MASS, SIG) \
+  KEntry<T> { N, 32 + CODE, ADV, MASS, SIG, &launch_staged<T, N, CODE, ADV, MASS, SIG> }
+#define GD_K3K_STAGED(T, N, CODE)                                                                             \
+  GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 0, kSigCyclic), GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 1, kSigCyclic),        \
+      GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 0, kSigConst), GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 1, kSigConst),      \
+      GD_K3K_STAGED_ENTRY(T, N, CODE, 0, 1, kSigConst)
+#define GD_K3K_STAGED_ALL(T, N)                                                                        \
+  GD_K3K_STAGED(T, N, 0), GD_K3K_STAGED(T, N, 1), GD_K3K_STAGED(T, N, 2), GD_K3K_STAGED(T, N, 3),          \
+      GD_K3K_STAGED(T, N, 4), GD_K3K_STAGED(T, N, 5), GD_K3K_STAGED(T, N, 6), GD_K3K_STAGED(T, N, 7)
 
 }  // namespace k3k
 }  // namespace gd

@@ -1,8 +1,9 @@
 """Every kernel variant (generic LDS-staged kernel, register-resident 3D
 kernel with 1, 2 or 4 lanes per element, Kronecker-factored 3D kernel with
-16 + lanes) against GenDiL's golden vectors and the oracle, and the
-autotuner's choice.  A pinned variant that does not serve a plan falls back
-to one that does, so every case must still be correct."""
+16 + lanes, and its LDS-staged forms 48..55) against GenDiL's golden vectors
+and the oracle, and the autotuner's choice.  A pinned variant that does not
+serve a plan falls back to one that does, so every case must still be
+correct."""
 import numpy as np
 import pytest
 import torch
@@ -17,7 +18,9 @@ pytestmark = pytest.mark.gpu
 ELIGIBLE = ["adv3d_c2_p3q4_per", "adv3d_c2_p3q4_bc0", "adv3d_c2_p3q4_nobc", "adv3d_trig_p3q4_bcpoly",
             "mass3d_p3q4_one", "adv3d_trig_p2q3_m1per", "adv3d_const_p2q4_per", "adv3d_bubble_p1q3",
             "adv3d_bubble_p2q4", "adv3d_bubble_p3q5"]
-KRON = {1: [17, 18], 2: [17], 3: [18, 20]}
+STAGED = list(range(48, 56))
+KRON = {1: [17, 18] + STAGED, 2: [17], 3: [18, 20] + STAGED}
+ALL_VARIANTS = (0, 1, 2, 4, 17, 18, 20, *STAGED)
 
 
 def variants(case):
@@ -41,7 +44,7 @@ def test_all_variants_match_golden(name, dtype):
         assert err <= (1e-12 if dtype == torch.float64 else 1e-5), (lanes, err)
 
 
-@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 18, 20])
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 18, 20, *STAGED])
 def test_variants_combined_operator_at_16_cubed(lanes):
     mesh = G.CartesianMesh.unit(3, 16, False)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
@@ -64,7 +67,7 @@ def test_autotune_picks_a_correct_variant():
     for dtype in (torch.float64, torch.float32):
         lanes, ms = op.plan.autotune(dtype, 0.0, 1.0, reps=3)
         print(f"autotune {dtype}: lanes={lanes} {ms:.4f} ms", flush=True)
-        assert lanes in (0, 1, 2, 4, 17, 18, 20) and ms > 0
+        assert lanes in ALL_VARIANTS and ms > 0
     x = G.fill_random(op.num_dofs, 1)
     y = torch.empty(op.num_dofs, dtype=torch.float64, device="cuda")
     op(torch.as_tensor(x, device="cuda"), y)
@@ -76,8 +79,10 @@ def test_kronecker_kernel_is_the_default_for_the_benchmark_field():
     mesh = G.CartesianMesh.unit(3, 8, True)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
     op = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(4), W.advection_c2())
-    assert op.plan.query_variant(torch.float64, 0.0, 1.0) == 20
-    assert op.plan.query_variant(torch.float32, 0.0, 1.0) == 20
+    # staged Kronecker kernel, outputs staged through LDS (16 + 32 + 1),
+    # FP32 in 128-thread blocks (+ 2)
+    assert op.plan.query_variant(torch.float64, 0.0, 1.0) == 49
+    assert op.plan.query_variant(torch.float32, 0.0, 1.0) == 51
     # a field whose sign changes along a face is served by the register kernel
     bad = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(4), beta_field("trig", 3))
     assert bad.plan.query_variant(torch.float64, 0.0, 1.0) in (1, 2, 4)
@@ -120,3 +125,55 @@ def test_kronecker_kernel_vs_oracle(case, dtype):
     err = rel_l2(y.double().cpu().numpy(), ref)
     print(f"rel l2 = {err:.3e}")
     assert err <= (1e-12 if dtype == torch.float64 else 1e-5)
+
+
+STAGED_MESHES = [
+    # (sizes, periodic): rows longer than a block, rows shorter than a block,
+    # a partial last block, a single-element axis
+    ((70, 3, 2), (True, False, True)),
+    ((5, 7, 3), (False, True, True)),
+    ((9, 1, 4), (True, True, False)),
+]
+
+
+@pytest.mark.parametrize("mesh_case", STAGED_MESHES, ids=["rows70", "rows5", "axis1"])
+@pytest.mark.parametrize("p", [1, 3])
+@pytest.mark.parametrize("kind", ["c2", "negc2", "const"])
+def test_staged_kronecker_variants_on_uneven_meshes(mesh_case, p, kind):
+    sizes, periodic = mesh_case
+    mesh = G.CartesianMesh(sizes, [1.0 / s for s in sizes], [0.0] * 3, periodic)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(p))
+    ir = G.MakeIntegrationRule(p + 1)
+    adv = negc2() if kind == "negc2" else beta_field(kind, 3)
+    for mc, ac, bc in ((0.0, 1.0, None), (0.6, 0.8, 0.0)):
+        op = G.MakeMassAdvectionOperator(fes, ir, adv, bc=bc, mass_coef=mc, adv_coef=ac)
+        x = G.fill_random(op.num_dofs, 21)
+        ref = oracle.Problem(fes, ir, mass_coef=mc, adv_coef=ac, adv=adv, bc=bc).apply(x)
+        for dtype in (torch.float64, torch.float32):
+            for v in STAGED:
+                print(f"staged {sizes} p={p} {kind} m={mc} {dtype} variant {v}", flush=True)
+                op.plan.set_variant(dtype, v)
+                assert op.plan.query_variant(dtype, mc, ac) == v
+                y = torch.empty(op.num_dofs, dtype=dtype, device="cuda")
+                op(torch.as_tensor(x, dtype=dtype, device="cuda"), y)
+                err = rel_l2(y.double().cpu().numpy(), ref)
+                assert err <= (1e-12 if dtype == torch.float64 else 1e-5), (v, err)
+
+
+def test_staged_kernel_unaligned_vectors_fall_back():
+    """The staged kernels move 16-byte chunks; vectors that are not 16-byte
+    aligned are served by the register Kronecker kernel instead."""
+    mesh = G.CartesianMesh.unit(3, 6, True)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    ir = G.MakeIntegrationRule(4)
+    op = G.MakeAdvectionOperator(fes, ir, W.advection_c2())
+    N = op.num_dofs
+    x = G.fill_random(N, 4)
+    ref = oracle.Problem(fes, ir, adv_coef=1.0, adv=W.advection_c2()).apply(x)
+    xb = torch.zeros(N + 1, dtype=torch.float64, device="cuda")
+    yb = torch.zeros(N + 1, dtype=torch.float64, device="cuda")
+    xb[1:] = torch.as_tensor(x, device="cuda")
+    op.plan.set_variant(torch.float64, 53)
+    op(xb[1:], yb[1:])
+    assert rel_l2(yb[1:].cpu().numpy(), ref) <= 1e-12
+    assert float(yb[0]) == 0.0
