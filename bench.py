@@ -45,6 +45,21 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of the default apply kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
+    copy); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k3k_staged_adv3d_64cubed_p3_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def time_applies(op, xs, ys, steps, warmup):
     import torch
 
@@ -238,7 +253,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("traffic_bytes_per_launch_f64") if n == 64 else None,
         },
     }
     if args.copy_calibration:
