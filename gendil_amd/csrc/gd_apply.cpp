@@ -272,8 +272,7 @@ int gd_plan_query_variant(const gd_plan* plan, int32_t precision_bytes, double m
   if (!plan || !variant || (precision_bytes != 4 && precision_bytes != 8))
     return fail(GD_ERR_INVALID, "gd_plan_query_variant: bad arguments");
   if (plan->padapt) {
-    const int v = plan->tuned_pa[precision_bytes == 8 ? 0 : 1];
-    *variant = v < 0 ? 0 : v;
+    *variant = gd::pa::resolved_variant(plan, plan->tuned_pa[precision_bytes == 8 ? 0 : 1]);
     return GD_OK;
   }
   const bool adv = adv_coef != 0.0, mass = mass_coef != 0.0;

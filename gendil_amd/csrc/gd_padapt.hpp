@@ -61,6 +61,8 @@ struct PParams {
   T affA[kMaxDim][kMaxDim];
   T affc[kMaxDim];
   int zero_axis_mask;  // bit a: beta_a == 0 identically (face flux vanishes)
+  int neg_mask;        // bit a: beta_a <= 0 on the whole mesh (Kronecker 6D kernel)
+  int kron_ok;         // the field suits the Kronecker 6D kernel (gd_pa6k.hpp)
   T sigma;             // constant
   int bc_mode;
   T bc_value;          // constant inflow value

@@ -26,6 +26,15 @@ template <typename T>
 int launch(int D, bool adv, int v, int grid, int it0, int it1, const void* pp, T mass_coef, T adv_coef, const T* x,
            T* y, hipStream_t s);
 
+// Launch the Kronecker 6D kernel (gd_pa6k.hpp): it0 blocks of one n=4 element,
+// it1 blocks of four n=3 elements, then blocks of four n=2 elements.
+template <typename T>
+int launch_kron(bool adv, int grid, int it0, int it1, const void* pp, T mass_coef, T adv_coef, const T* x, T* y,
+                hipStream_t s);
+
+// Variant an apply with variant knob v (-1: default) resolves to.
+int resolved_variant(const gd_plan* plan, int v);
+
 // Plan-side state (element buckets, device parameter copies).
 struct State;
 void destroy_state(State* s);

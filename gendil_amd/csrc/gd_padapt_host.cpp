@@ -19,6 +19,7 @@ struct State {
   int count[3] = {0, 0, 0};
   void* params[2] = {nullptr, nullptr};  // device PParams<double>, PParams<float>
   int version[2] = {-1, -1};
+  bool kron_ok = false;  // the current field suits the Kronecker 6D kernel
 };
 
 void destroy_state(State* s)
@@ -30,11 +31,13 @@ void destroy_state(State* s)
   delete s;
 }
 
-int num_variants(int D) { return D == 6 ? 2 : 1; }
+// 6D: 0, 1 = quadrature kernel layouts, 2 = Kronecker kernel (phase-space
+// fields, falls back to 0 for other fields)
+int num_variants(int D) { return D == 6 ? 3 : 1; }
 
 Variant variant(int D, int v)
 {
-  if (D == 6) return v == 1 ? Variant{256, 256, 64} : Variant{256, 64, 64};
+  if (D == 6) return v == 1 ? Variant{256, 256, 64} : (v == 2 ? Variant{256, 64, 64} : Variant{256, 64, 64});
   return Variant{64, 64, 64};
 }
 
@@ -86,6 +89,26 @@ PParams<T> host_params(const gd_plan* plan, const State& st)
       zero = v == 0.0;
     }
     if (zero) k.zero_axis_mask |= 1 << c;
+  }
+  // Kronecker 6D kernel: beta_c = a_c X_pc + b_c, pc = (c + 3) mod 6, of one
+  // sign over the mesh (its extremes are at the ends of the pc-range)
+  k.kron_ok = D == 6 ? 1 : 0;
+  for (int c = 0; c < D && k.kron_ok; ++c) {
+    const int pc = (c + 3) % 6;
+    double a = 0.0, b = 0.0;
+    if (plan->beta_kind == GD_BETA_AFFINE) {
+      for (int d = 0; d < D; ++d)
+        if (d != pc && plan->affA[c][d] != 0.0) k.kron_ok = 0;
+      a = plan->affA[c][pc];
+      b = plan->affc[c];
+    } else {
+      constant_field(plan->beta_host[c], &b);
+    }
+    const double lo = plan->desc.origin[pc], hi = lo + plan->desc.sizes[pc] * plan->desc.h[pc];
+    const double v0 = a * lo + b, v1 = a * hi + b;
+    if (v0 >= 0.0 && v1 >= 0.0) continue;
+    if (v0 <= 0.0 && v1 <= 0.0) k.neg_mask |= 1 << c;
+    else k.kron_ok = 0;
   }
   double sigma = 1.0, g = 0.0;
   constant_field(plan->sigma_host, &sigma);
@@ -176,8 +199,15 @@ int apply(const gd_plan* plan, int v, T mass_coef, T adv_coef, const T* x, T* y,
     const PParams<T> hp = host_params<T>(plan, st);
     GD_HIP(hipMemcpy(st.params[slot], &hp, sizeof(hp), hipMemcpyHostToDevice));
     st.version[slot] = plan->coef_version;
+    st.kron_ok = hp.kron_ok != 0;
   }
-  if (v < 0 || v >= num_variants(plan->dim)) v = 0;
+  const bool kron = plan->dim == 6 && st.kron_ok;
+  if (v < 0) v = kron ? 2 : 0;
+  if (v >= num_variants(plan->dim) || (v == 2 && !kron)) v = 0;
+  if (v == 2) {
+    const int it0 = st.count[0], it1 = (st.count[1] + 3) / 4, it2 = (st.count[2] + 3) / 4;
+    return launch_kron<T>(adv_coef != T(0), it0 + it1 + it2, it0, it1, st.params[slot], mass_coef, adv_coef, x, y, s);
+  }
   const Variant var = variant(plan->dim, v);
   const int tpe[3] = {var.tpe4, var.tpe3, var.tpe2};
   int items[3];
@@ -188,6 +218,21 @@ int apply(const gd_plan* plan, int v, T mass_coef, T adv_coef, const T* x, T* y,
   const int grid = items[0] + items[1] + items[2];
   return launch<T>(plan->dim, adv_coef != T(0), v, grid, items[0], items[1], st.params[slot], mass_coef, adv_coef, x,
                    y, s);
+}
+
+int resolved_variant(const gd_plan* plan, int v)
+{
+  const char* why = nullptr;
+  if (!eligible(plan, &why)) return 0;
+  if (plan->pa_state->version[0] != plan->coef_version && plan->pa_state->version[1] != plan->coef_version) {
+    // parameters not built yet for this field: decide from a host copy
+    const PParams<double> hp = host_params<double>(plan, *plan->pa_state);
+    plan->pa_state->kron_ok = hp.kron_ok != 0;
+  }
+  const bool kron = plan->dim == 6 && plan->pa_state->kron_ok;
+  if (v < 0) v = kron ? 2 : 0;
+  if (v >= num_variants(plan->dim) || (v == 2 && !kron)) v = 0;
+  return v;
 }
 
 template int apply<double>(const gd_plan*, int, double, double, const double*, double*, hipStream_t);

@@ -119,11 +119,98 @@ def test_padapt_uniform_orders_match_uniform_kernels():
 
 
 def test_padapt_autotune_keeps_results():
+    # the quadrature layouts (0, 1) agree bitwise; the Kronecker kernel (2)
+    # regroups the same exact integrals, so it agrees to rounding
     fes, adv = W.phase_space_6d_padaptive(3, seed=16)
     K, x, y0 = run(fes, adv, 0.0, 1.0, None, torch.float32)
     chosen, ms = K.plan.autotune(torch.float32, 0.0, 1.0, reps=3)
     print(f"autotune chose variant {chosen} ({ms:.3f} ms)")
+    assert chosen in (0, 1, 2)
     xt = torch.as_tensor(x, device="cuda").float()
     y = torch.empty_like(xt)
     K(xt, y)
-    assert np.array_equal(y.double().cpu().numpy(), y0)
+    assert rel_l2(y.double().cpu().numpy(), y0) <= F32_TOL
+
+
+def paired_field(signs=(1, -1, 1, -1, 1, 0)):
+    """beta_c = a_c X_{(c+3) mod 6} + b_c with the given signs over [0,1]^6
+    (the structure the Kronecker 6D kernel serves); 0 = zero component."""
+    A = np.zeros((6, 6))
+    c0 = np.zeros(6)
+    slopes = [0.5, 1.0, 0.0, 0.0, 0.25, 0.0]
+    offsets = [0.2, 0.1, 0.3, 0.4, 0.1, 0.0]
+    for c, sg in enumerate(signs):
+        A[c, (c + 3) % 6] = sg * slopes[c]
+        c0[c] = sg * offsets[c]
+    return AffineField(A, c0)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, F64_TOL), (torch.float32, F32_TOL)])
+def test_padapt_6d_kronecker_is_default_and_matches_oracle(dtype, tol):
+    print(f"Kronecker 6D 3^6 phase space {dtype}", flush=True)
+    fes, adv = W.phase_space_6d_padaptive(3, seed=13)
+    K = G.MakeAdvectionOperator(fes, None, adv)
+    assert K.plan.query_variant(dtype, 0.0, 1.0) == 2
+    _, x, y = run(fes, adv, 0.0, 1.0, None, dtype)
+    ref = oracle_apply(fes, adv, 0.0, 1.0, None, x)
+    err = rel_l2(y, ref)
+    print(f"rel l2 = {err:.3e}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, F64_TOL), (torch.float32, F32_TOL)])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_padapt_6d_kronecker_mixed_signs_mass_inflow(dtype, tol, periodic):
+    mesh = G.CartesianMesh([3] * 6, [1 / 3] * 6, [0.0] * 6, periodic)
+    orders = W.padaptive_orders(mesh.num_cells, 17)
+    fes = G.MakePAdaptiveSpace(mesh, orders, orders + 1)
+    adv = paired_field()
+    bc = None if periodic else 0.75
+    print(f"Kronecker 6D mixed signs periodic={periodic} {dtype}", flush=True)
+    K = G.MakeMassAdvectionOperator(fes, None, adv, bc=bc, mass_coef=1.0, adv_coef=0.5)
+    assert K.plan.query_variant(dtype, 1.0, 0.5) == 2
+    _, x, y = run(fes, adv, 1.0, 0.5, bc, dtype, seed=18)
+    ref = oracle_apply(fes, adv, 1.0, 0.5, bc, x)
+    err = rel_l2(y, ref)
+    print(f"rel l2 = {err:.3e}")
+    assert err <= tol
+
+
+def test_padapt_6d_kronecker_mass_only():
+    fes, adv = W.phase_space_6d_padaptive(3, seed=19)
+    _, x, y = run(fes, adv, 1.0, 0.0, None, torch.float64)
+    ref = oracle_apply(fes, adv, 1.0, 0.0, None, x)
+    assert rel_l2(y, ref) <= F64_TOL
+
+
+def test_padapt_6d_unpaired_field_uses_quadrature_kernel():
+    mesh = G.CartesianMesh([3] * 6, [1 / 3] * 6, [0.0] * 6, True)
+    orders = W.padaptive_orders(mesh.num_cells, 15)
+    fes = G.MakePAdaptiveSpace(mesh, orders, orders + 1)
+    A = np.zeros((6, 6))
+    A[0, 3] = A[1, 4] = A[2, 5] = 1.0
+    A[3, 1] = -0.5  # beta_3 depends on X1, not on its partner X0
+    K = G.MakeAdvectionOperator(fes, None, AffineField(A, np.zeros(6)))
+    assert K.plan.query_variant(torch.float64, 0.0, 1.0) == 0
+    # a sign change inside the mesh is not served either
+    K2 = G.MakeAdvectionOperator(fes, None, paired_field(signs=(1, 1, 1, 1, 1, 1)))
+    assert K2.plan.query_variant(torch.float64, 0.0, 1.0) == 2
+    Ac = np.zeros((6, 6))
+    Ac[0, 3] = 1.0
+    K3 = G.MakeAdvectionOperator(fes, None, AffineField(Ac, np.array([-0.5, 0, 0, 0, 0, 0])))
+    assert K3.plan.query_variant(torch.float64, 0.0, 1.0) == 0
+
+
+def test_padapt_6d_kronecker_uniform_orders_match_uniform_kernel():
+    mesh = G.CartesianMesh([3] * 6, [1 / 3] * 6, [0.0] * 6, True)
+    E = mesh.num_cells
+    adv = paired_field()
+    pfes = G.MakePAdaptiveSpace(mesh, np.full(E, 2), np.full(E, 3))
+    K, x, y = run(pfes, adv, 0.5, 1.0, None, torch.float64)
+    assert K.plan.query_variant(torch.float64, 0.5, 1.0) == 2
+    ufes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(2))
+    U = G.MakeMassAdvectionOperator(ufes, G.MakeIntegrationRule(3), adv, bc=None, mass_coef=0.5, adv_coef=1.0)
+    xt = torch.as_tensor(x, device="cuda")
+    yu = torch.empty_like(xt)
+    U(xt, yu)
+    assert rel_l2(y, yu.cpu().numpy()) <= 1e-12
