@@ -37,12 +37,40 @@
 #include "gd_consttab.hpp"
 #include "gd_padapt.hpp"
 
+#include <utility>
+
 namespace gd {
 namespace pa6 {
 
 using pa::PParams;
 
 constexpr int ipw(int b, int e) { return e == 0 ? 1 : b * ipw(b, e - 1); }
+
+// LDS position of element entry idx.  For N = 4 the low six bits (the bank)
+// are XOR-ed with (i4, i3, i3) in their three 2-bit fields: the 64 lanes of a
+// wave, which in the three pair phases run over (i0,i1,i3), (i0,i2,i3) and
+// (i1,i2,i4) with the rest fixed, then hit 64 different banks in every phase
+// (with the plain layout phase (0,3) is 4-way conflicted).  A bijection.
+template <int N>
+__device__ __forceinline__ int swz(int idx)
+{
+  if constexpr (N == 4) {
+    const int i3 = (idx >> 6) & 3, i4 = (idx >> 8) & 3;
+    return idx ^ (i4 | (i3 << 2) | (i3 << 4));
+  } else {
+    return idx;
+  }
+}
+
+// Face scratch: the five tangential axes with strides 1, 4, 16, 64, 256
+// (1024 values); with PAD 1, 5, 20, 80, 320 (1280 values: the first axis
+// padded to 5 so a sweep along it, whose lanes step the second axis, has no
+// bank conflicts; measured slower overall, as the larger LDS footprint costs
+// a resident block per CU).
+template <bool PAD>
+constexpr int scr_words() { return PAD ? 1280 : 1024; }
+template <bool PAD>
+constexpr int sstride(int k) { return k == 0 ? 1 : (PAD ? 5 : 4) * ipw(4, k - 1); }
 
 // 1D matrices between an NO-node and an NI-node GLL basis on the Gauss rule of
 // max(NO, NI) points: mass Mx, xi-weighted mass Mx1; for NO == NI also the
@@ -76,6 +104,15 @@ struct Mix {
   }
   static constexpr Mats m = make();
 };
+
+template <int I, int NN, typename F>
+__device__ __forceinline__ void kfor6(F&& f)
+{
+  if constexpr (I < NN) {
+    f(std::integral_constant<int, I>{});
+    kfor6<I + 1, NN>(f);
+  }
+}
 
 template <int G>
 __device__ __forceinline__ void gsync()
@@ -231,7 +268,7 @@ __device__ __forceinline__ void read_slices(const T* X, int a, int tid, int G, T
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) S[s][i][j] = X[b + i * ipw(N, a) + j * ipw(N, a + 3)];
+      for (int j = 0; j < N; ++j) S[s][i][j] = X[swz<N>(b + i * ipw(N, a) + j * ipw(N, a + 3))];
   }
 }
 
@@ -247,36 +284,32 @@ __device__ __forceinline__ void write_slices(T* X, int a, int tid, int G, const 
 #pragma unroll
       for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < N; ++j) X[b + i * ipw(N, a) + j * ipw(N, a + 3)] = S[s][i][j];
+        for (int j = 0; j < N; ++j) X[swz<N>(b + i * ipw(N, a) + j * ipw(N, a + 3))] = S[s][i][j];
     }
   }
 }
 
-// One inflow face of an N-node element from an NB-node neighbour (or inflow
-// data g): convert the slab in the wave's scratch (stride 4 per tangential
-// axis); one wave (64 lanes) takes part.
-template <typename T, int N, int NB>
-__device__ __forceinline__ void face_convert(const T* __restrict__ xnb, bool data, T g, int c, const Comp<T>& cc,
-                                             T* scr, int lane)
+// The five tangential axes of a face normal to C, ascending.
+template <int C>
+struct FaceAxes {
+  static constexpr int ax(int k) { return k < C ? k : k + 1; }
+};
+
+// One inflow face (normal to C) of an N-node element from an NB-node
+// neighbour (or inflow data g): convert the slab in the wave's scratch; one
+// wave (64 lanes) takes part.
+template <typename T, int N, int NB, int C>
+__device__ __forceinline__ void face_convert(const T* __restrict__ xnb, bool data, T g, const Comp<T>& cc, T* scr,
+                                             int lane)
 {
   constexpr int WL = 64;
   using MX = Mix<N, NB>;
-  const int pc = (c + 3) % 6;
-  int ax[5];
-  {
-    int k = 0;
-#pragma unroll
-    for (int d = 0; d < 6; ++d)
-      if (d != c) ax[k++] = d;
-  }
+  using FA = FaceAxes<C>;
+  constexpr int PC = (C + 3) % 6;
   // the neighbour's face-node slab (or the constant inflow data)
   {
     constexpr int NS = ipw(NB, 5);
     const int ns = cc.neg ? 0 : NB - 1;  // neighbour node at the shared face
-    int sc = 1;
-#pragma unroll
-    for (int d = 0; d < 6; ++d)
-      if (d < c) sc *= NB;
 #pragma unroll 1
     for (int v = lane; v < NS; v += WL) {
       int r = v, src = 0, dst = 0;
@@ -284,25 +317,27 @@ __device__ __forceinline__ void face_convert(const T* __restrict__ xnb, bool dat
       for (int k = 0; k < 5; ++k) {
         const int dg = r % NB;
         r /= NB;
-        int stride = 1;
-#pragma unroll
-        for (int d = 0; d < 6; ++d)
-          if (d < ax[k]) stride *= NB;
-        src += dg * stride;
-        dst += dg * ipw(4, k);
+        src += dg * ipw(NB, FA::ax(k));
+        dst += dg * sstride<false>(k);
       }
-      scr[dst] = data ? xnb[src + ns * sc] : g;
+      scr[dst] = data ? xnb[src + ns * ipw(NB, C)] : g;
     }
   }
   gsync<64>();
   // five in-place sweeps: slot k goes from NB to N values; the face
   // coefficient enters with the last one
   const T coef = cc.neg ? cc.kt : -cc.kt;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    const bool beta_axis = ax[k] == pc;
+  kfor6<0, 5>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    constexpr bool BETA = FA::ax(k) == PC;
+    constexpr int P = ipw(N, k) * ipw(NB, 4 - k);
     const T fin = k == 4 ? coef : T(1);
-    const int P = ipw(N, k) * ipw(NB, 4 - k);
+    T mat[N][NB];
+#pragma unroll
+    for (int ii = 0; ii < N; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj)
+        mat[ii][jj] = fin * (BETA ? cc.b0 * (T)MX::m.M[ii][jj] + cc.b1 * (T)MX::m.M1[ii][jj] : (T)MX::m.M[ii][jj]);
 #pragma unroll 1
     for (int q = lane; q < P; q += WL) {
       int r = q, base = 0;
@@ -310,45 +345,32 @@ __device__ __forceinline__ void face_convert(const T* __restrict__ xnb, bool dat
       for (int j = 0; j < 5; ++j) {
         if (j == k) continue;
         const int ext = j < k ? N : NB;
-        base += (r % ext) * ipw(4, j);
+        base += (r % ext) * sstride<false>(j);
         r /= ext;
       }
       T in[NB];
 #pragma unroll
-      for (int jj = 0; jj < NB; ++jj) in[jj] = scr[base + jj * ipw(4, k)];
+      for (int jj = 0; jj < NB; ++jj) in[jj] = scr[base + jj * sstride<false>(k)];
 #pragma unroll
       for (int ii = 0; ii < N; ++ii) {
         T sum = T(0);
 #pragma unroll
-        for (int jj = 0; jj < NB; ++jj) {
-          const T m = beta_axis ? cc.b0 * (T)MX::m.M[ii][jj] + cc.b1 * (T)MX::m.M1[ii][jj] : (T)MX::m.M[ii][jj];
-          sum += m * in[jj];
-        }
-        scr[base + ii * ipw(4, k)] = fin * sum;
+        for (int jj = 0; jj < NB; ++jj) sum += mat[ii][jj] * in[jj];
+        scr[base + ii * sstride<false>(k)] = sum;
       }
     }
     gsync<64>();
-  }
+  });
 }
 
-// Add a converted face (scratch, stride 4) at the element's inflow node layer.
-template <typename T, int N>
-__device__ __forceinline__ void face_add(int c, const Comp<T>& cc, const T* scr, T* Y, int lane)
+// Add a converted face (scratch) at the element's inflow node layer.
+template <typename T, int N, int C>
+__device__ __forceinline__ void face_add(const Comp<T>& cc, const T* scr, T* Y, int lane)
 {
   constexpr int WL = 64;
   constexpr int NS = ipw(N, 5);
-  int ax[5];
-  {
-    int k = 0;
-#pragma unroll
-    for (int d = 0; d < 6; ++d)
-      if (d != c) ax[k++] = d;
-  }
+  using FA = FaceAxes<C>;
   const int ni = cc.neg ? N - 1 : 0;
-  int sc = 1;
-#pragma unroll
-  for (int d = 0; d < 6; ++d)
-    if (d < c) sc *= N;
 #pragma unroll 1
   for (int v = lane; v < NS; v += WL) {
     int r = v, dst = 0, src = 0;
@@ -356,43 +378,70 @@ __device__ __forceinline__ void face_add(int c, const Comp<T>& cc, const T* scr,
     for (int k = 0; k < 5; ++k) {
       const int dg = r % N;
       r /= N;
-      int stride = 1;
-#pragma unroll
-      for (int d = 0; d < 6; ++d)
-        if (d < ax[k]) stride *= N;
-      dst += dg * stride;
-      src += dg * ipw(4, k);
+      dst += dg * ipw(N, FA::ax(k));
+      src += dg * sstride<false>(k);
     }
-    Y[dst + ni * sc] += scr[src];
+    Y[swz<N>(dst + ni * ipw(N, C))] += scr[src];
   }
 }
 
-// Convert inflow face c into scr; false when the face has no term (boundary
+// Convert inflow face C into scr; false when the face has no term (boundary
 // without inflow data).
-template <typename T, int N>
-__device__ __forceinline__ bool face_dispatch(const PParams<T>& p, const T* __restrict__ x, int64_t e, const int* sub,
-                                              int c, const Comp<T>& cc, T* scr, int lane)
+template <typename T, int N, int C>
+__device__ __forceinline__ bool face_dispatch_c(const PParams<T>& p, const T* __restrict__ x, int64_t e,
+                                                const int* sub, const Comp<T>& cc, T* scr, int lane)
 {
-  const int64_t nb = pa::neighbour<T, 6>(p, e, sub, cc.neg ? c + 6 : c);
+  const int64_t nb = pa::neighbour<T, 6>(p, e, sub, cc.neg ? C + 6 : C);
   if (nb < 0) {
     if (p.bc_mode != GD_BC_INFLOW) return false;
-    face_convert<T, N, N>(nullptr, false, p.bc_value, c, cc, scr, lane);
+    face_convert<T, N, N, C>(nullptr, false, p.bc_value, cc, scr, lane);
     return true;
   }
   const int nbn = p.order[nb] + 1;
   const T* xn = x + p.dof_offset[nb];
-  if (nbn == 2) face_convert<T, N, 2>(xn, true, T(0), c, cc, scr, lane);
-  else if (nbn == 3) face_convert<T, N, 3>(xn, true, T(0), c, cc, scr, lane);
-  else face_convert<T, N, 4>(xn, true, T(0), c, cc, scr, lane);
+  if (nbn == 2) face_convert<T, N, 2, C>(xn, true, T(0), cc, scr, lane);
+  else if (nbn == 3) face_convert<T, N, 3, C>(xn, true, T(0), cc, scr, lane);
+  else face_convert<T, N, 4, C>(xn, true, T(0), cc, scr, lane);
   return true;
 }
 
+template <typename T, int N>
+__device__ __forceinline__ bool face_dispatch(const PParams<T>& p, const T* __restrict__ x, int64_t e, const int* sub,
+                                              int c, const Comp<T>& cc, T* scr, int lane)
+{
+  switch (c) {
+    case 0: return face_dispatch_c<T, N, 0>(p, x, e, sub, cc, scr, lane);
+    case 1: return face_dispatch_c<T, N, 1>(p, x, e, sub, cc, scr, lane);
+    case 2: return face_dispatch_c<T, N, 2>(p, x, e, sub, cc, scr, lane);
+    case 3: return face_dispatch_c<T, N, 3>(p, x, e, sub, cc, scr, lane);
+    case 4: return face_dispatch_c<T, N, 4>(p, x, e, sub, cc, scr, lane);
+    default: return face_dispatch_c<T, N, 5>(p, x, e, sub, cc, scr, lane);
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void face_add_any(int c, const Comp<T>& cc, const T* scr, T* Y, int lane)
+{
+  switch (c) {
+    case 0: face_add<T, N, 0>(cc, scr, Y, lane); break;
+    case 1: face_add<T, N, 1>(cc, scr, Y, lane); break;
+    case 2: face_add<T, N, 2>(cc, scr, Y, lane); break;
+    case 3: face_add<T, N, 3>(cc, scr, Y, lane); break;
+    case 4: face_add<T, N, 4>(cc, scr, Y, lane); break;
+    default: face_add<T, N, 5>(cc, scr, Y, lane); break;
+  }
+}
+
 // LDS words per element slot: two element buffers, plus a face scratch for
-// the wave-per-element classes (n = 4 uses the second buffer as four scratches)
+// the wave-per-element classes (n = 4 uses the second buffer and what follows
+// it as four scratches)
 template <int N>
 constexpr int slot_words()
 {
-  return 2 * ipw(N, 6) + (N == 4 ? 0 : 1024);
+  // the second buffer is free once the last phase has read it, and takes the
+  // face scratch (four of them for the block-per-element class)
+  const int scr = N == 4 ? 4 * scr_words<false>() : scr_words<false>();
+  return ipw(N, 6) + (ipw(N, 6) > scr ? ipw(N, 6) : scr);
 }
 constexpr int block_words()
 {
@@ -412,7 +461,7 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
   int sub[6];
   pa::decode<T, 6>(p, e, sub);
 #pragma unroll 1
-  for (int i = tid; i < N6; i += G) X0[i] = x[off + i];
+  for (int i = tid; i < N6; i += G) X0[swz<N>(i)] = x[off + i];
 
   // per-element component data
   Comp<T> cp[6];
@@ -513,12 +562,12 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
   // ---- inflow faces ----
   if constexpr (ADV) {
     if constexpr (G == 64) {
-      T* scr = slot + 2 * N6;
+      T* scr = X1;
 #pragma unroll 1
       for (int c = 0; c < 6; ++c)
         if ((act >> c) & 1) {
           if (face_dispatch<T, N>(p, x, e, sub, c, cp[c], scr, tid)) {
-            face_add<T, N>(c, cp[c], scr, X0, tid);
+            face_add_any<T, N>(c, cp[c], scr, X0, tid);
             gsync<64>();
           }
         }
@@ -526,7 +575,7 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
       // four waves convert one face each per round (the second buffer is
       // free now: four scratches), then add them in face order
       const int w = tid / 64, lane = tid % 64;
-      T* scr = X1 + w * 1024;
+      T* scr = X1 + w * scr_words<false>();
       int faces[6], nf = 0;
 #pragma unroll
       for (int c = 0; c < 6; ++c)
@@ -539,14 +588,14 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
         __syncthreads();
 #pragma unroll 1
         for (int turn = 0; turn < 4; ++turn) {
-          if (turn == w && mine) face_add<T, N>(faces[k], cp[faces[k]], scr, X0, lane);
+          if (turn == w && mine) face_add_any<T, N>(faces[k], cp[faces[k]], scr, X0, lane);
           __syncthreads();
         }
       }
     }
   }
 #pragma unroll 1
-  for (int i = tid; i < N6; i += G) y[off + i] = X0[i];
+  for (int i = tid; i < N6; i += G) y[off + i] = X0[swz<N>(i)];
 }
 
 // One launch: it0 blocks of one n = 4 element each, then it1 blocks of four
