@@ -435,13 +435,17 @@ __device__ __forceinline__ void face_add_any(int c, const Comp<T>& cc, const T* 
 // LDS words per element slot: two element buffers, plus a face scratch for
 // the wave-per-element classes (n = 4 uses the second buffer and what follows
 // it as four scratches)
+// per-element component table at the end of a slot: kt, b0, b1, flags
+// (bit 0 own, bit 1 neg) for the six components
+constexpr int kCompWords = 24;
+
 template <int N>
 constexpr int slot_words()
 {
   // the second buffer is free once the last phase has read it, and takes the
   // face scratch (four of them for the block-per-element class)
   const int scr = N == 4 ? 4 * scr_words<false>() : scr_words<false>();
-  return ipw(N, 6) + (ipw(N, 6) > scr ? ipw(N, 6) : scr);
+  return ipw(N, 6) + (ipw(N, 6) > scr ? ipw(N, 6) : scr) + kCompWords;
 }
 constexpr int block_words()
 {
@@ -560,35 +564,57 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
   gsync<G>();
 
   // ---- inflow faces ----
+  // (the component is picked by selects, so the per-component data is never
+  // indexed dynamically, which would put it on the stack)
+  // (faces index the components at run time: they read them from a small
+  // LDS table, so the per-component registers are never indexed dynamically,
+  // which would put them on the stack)
+  T* ctab = slot + slot_words<N>() - kCompWords;
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      ctab[4 * k] = cp[k].kt;
+      ctab[4 * k + 1] = cp[k].b0;
+      ctab[4 * k + 2] = cp[k].b1;
+      ctab[4 * k + 3] = T((cp[k].own ? 1 : 0) + (cp[k].neg ? 2 : 0));
+    }
+  }
+  gsync<G>();
+  auto pick = [&](int c) -> Comp<T> {
+    const int f = (int)ctab[4 * c + 3];
+    return Comp<T>{ctab[4 * c], ctab[4 * c + 1], ctab[4 * c + 2], (f & 1) != 0, (f & 2) != 0};
+  };
   if constexpr (ADV) {
     if constexpr (G == 64) {
       T* scr = X1;
 #pragma unroll 1
-      for (int c = 0; c < 6; ++c)
-        if ((act >> c) & 1) {
-          if (face_dispatch<T, N>(p, x, e, sub, c, cp[c], scr, tid)) {
-            face_add_any<T, N>(c, cp[c], scr, X0, tid);
-            gsync<64>();
-          }
+      for (int c = 0; c < 6; ++c) {
+        if (!((act >> c) & 1)) continue;
+        const Comp<T> cc = pick(c);
+        if (face_dispatch<T, N>(p, x, e, sub, c, cc, scr, tid)) {
+          face_add_any<T, N>(c, cc, scr, X0, tid);
+          gsync<64>();
         }
+      }
     } else {
       // four waves convert one face each per round (the second buffer is
       // free now: four scratches), then add them in face order
       const int w = tid / 64, lane = tid % 64;
       T* scr = X1 + w * scr_words<false>();
-      int faces[6], nf = 0;
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-        if ((act >> c) & 1) faces[nf++] = c;
+      const int nf = __builtin_popcount(act);
 #pragma unroll 1
       for (int r0 = 0; r0 < nf; r0 += 4) {
-        const int k = r0 + w;
-        bool mine = false;
-        if (k < nf) mine = face_dispatch<T, N>(p, x, e, sub, faces[k], cp[faces[k]], scr, lane);
+        // this wave's face: the (r0 + w)-th active component
+        int c = -1;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          if (((act >> k) & 1) && __builtin_popcount(act & ((1 << k) - 1)) == r0 + w) c = k;
+        const Comp<T> cc = pick(c < 0 ? 0 : c);
+        const bool mine = c >= 0 && face_dispatch<T, N>(p, x, e, sub, c, cc, scr, lane);
         __syncthreads();
 #pragma unroll 1
         for (int turn = 0; turn < 4; ++turn) {
-          if (turn == w && mine) face_add_any<T, N>(faces[k], cp[faces[k]], scr, X0, lane);
+          if (turn == w && mine) face_add_any<T, N>(c, cc, scr, X0, lane);
           __syncthreads();
         }
       }
@@ -601,7 +627,7 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
 // One launch: it0 blocks of one n = 4 element each, then it1 blocks of four
 // n = 3 elements (a wave each), then blocks of four n = 2 elements.
 template <typename T, bool ADV>
-__global__ void __launch_bounds__(256) pa6k_kernel(const PParams<T>* __restrict__ pp, int it0, int it1, T mass_coef,
+__global__ void __launch_bounds__(256, 4) pa6k_kernel(const PParams<T>* __restrict__ pp, int it0, int it1, T mass_coef,
                                                    T adv_coef, const T* __restrict__ x, T* __restrict__ y)
 {
   const PParams<T>& p = *pp;
