@@ -388,17 +388,15 @@ __device__ __forceinline__ void face_add(const Comp<T>& cc, const T* scr, T* Y, 
 // Convert inflow face C into scr; false when the face has no term (boundary
 // without inflow data).
 template <typename T, int N, int C>
-__device__ __forceinline__ bool face_dispatch_c(const PParams<T>& p, const T* __restrict__ x, int64_t e,
-                                                const int* sub, const Comp<T>& cc, T* scr, int lane)
+__device__ __forceinline__ bool face_dispatch_c(const PParams<T>& p, const T* __restrict__ x, int nbn,
+                                                long long off, bool data, const Comp<T>& cc, T* scr, int lane)
 {
-  const int64_t nb = pa::neighbour<T, 6>(p, e, sub, cc.neg ? C + 6 : C);
-  if (nb < 0) {
-    if (p.bc_mode != GD_BC_INFLOW) return false;
+  if (nbn == 0) return false;
+  if (!data) {
     face_convert<T, N, N, C>(nullptr, false, p.bc_value, cc, scr, lane);
     return true;
   }
-  const int nbn = p.order[nb] + 1;
-  const T* xn = x + p.dof_offset[nb];
+  const T* xn = x + off;
   if (nbn == 2) face_convert<T, N, 2, C>(xn, true, T(0), cc, scr, lane);
   else if (nbn == 3) face_convert<T, N, 3, C>(xn, true, T(0), cc, scr, lane);
   else face_convert<T, N, 4, C>(xn, true, T(0), cc, scr, lane);
@@ -406,16 +404,16 @@ __device__ __forceinline__ bool face_dispatch_c(const PParams<T>& p, const T* __
 }
 
 template <typename T, int N>
-__device__ __forceinline__ bool face_dispatch(const PParams<T>& p, const T* __restrict__ x, int64_t e, const int* sub,
-                                              int c, const Comp<T>& cc, T* scr, int lane)
+__device__ __forceinline__ bool face_dispatch(const PParams<T>& p, const T* __restrict__ x, int c, int nbn,
+                                              long long off, bool data, const Comp<T>& cc, T* scr, int lane)
 {
   switch (c) {
-    case 0: return face_dispatch_c<T, N, 0>(p, x, e, sub, cc, scr, lane);
-    case 1: return face_dispatch_c<T, N, 1>(p, x, e, sub, cc, scr, lane);
-    case 2: return face_dispatch_c<T, N, 2>(p, x, e, sub, cc, scr, lane);
-    case 3: return face_dispatch_c<T, N, 3>(p, x, e, sub, cc, scr, lane);
-    case 4: return face_dispatch_c<T, N, 4>(p, x, e, sub, cc, scr, lane);
-    default: return face_dispatch_c<T, N, 5>(p, x, e, sub, cc, scr, lane);
+    case 0: return face_dispatch_c<T, N, 0>(p, x, nbn, off, data, cc, scr, lane);
+    case 1: return face_dispatch_c<T, N, 1>(p, x, nbn, off, data, cc, scr, lane);
+    case 2: return face_dispatch_c<T, N, 2>(p, x, nbn, off, data, cc, scr, lane);
+    case 3: return face_dispatch_c<T, N, 3>(p, x, nbn, off, data, cc, scr, lane);
+    case 4: return face_dispatch_c<T, N, 4>(p, x, nbn, off, data, cc, scr, lane);
+    default: return face_dispatch_c<T, N, 5>(p, x, nbn, off, data, cc, scr, lane);
   }
 }
 
@@ -435,9 +433,16 @@ __device__ __forceinline__ void face_add_any(int c, const Comp<T>& cc, const T* 
 // LDS words per element slot: two element buffers, plus a face scratch for
 // the wave-per-element classes (n = 4 uses the second buffer and what follows
 // it as four scratches)
-// per-element component table at the end of a slot: kt, b0, b1, flags
-// (bit 0 own, bit 1 neg) for the six components
-constexpr int kCompWords = 24;
+// Per-element component record (LDS, at the end of a slot): coefficients,
+// flags, and the inflow neighbour (order, DoF offset).
+template <typename T>
+struct alignas(8) CompRec {
+  T kt, b0, b1;
+  int flags;      // bit 0 own (outflow face term), bit 1 neg, bit 3 inflow data instead of a neighbour
+  int nbn;        // neighbour nodes per dimension (0: no face term)
+  long long off;  // neighbour DoF offset
+};
+constexpr int kCompWords = 6 * 48 / 4;  // 4-byte words, enough for six records of either precision
 
 template <int N>
 constexpr int slot_words()
@@ -485,6 +490,37 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
     cp[c].own = dn >= 0 || p.bc_mode == GD_BC_INFLOW;
   }
   const T km = mass_coef * p.detJ * p.sigma;
+  // component records for the face pass: lane c computes component c from
+  // the parameters (global memory) alone, and the neighbour's order and
+  // offset loads complete during the cell phases
+  CompRec<T>* ctab = reinterpret_cast<CompRec<T>*>(slot + slot_words<N>() - kCompWords * 4 / (int)sizeof(T));
+  if (ADV && tid < 6) {
+    const int c = tid, pc = (c + 3) % 6;
+    CompRec<T> r{T(0), T(0), T(0), 0, 0, 0};
+    if (!((p.zero_axis_mask >> c) & 1)) {
+      int64_t st = 1;
+      for (int d = 0; d < 6; ++d)
+        if (d < pc) st *= p.M[d];
+      const int spc = (int)((e / st) % p.M[pc]);
+      const T a = p.affA[c][pc];
+      r.kt = adv_coef * p.detJ * p.invh[c];
+      r.b0 = a * (p.origin[pc] + p.h[pc] * (T)spc) + p.affc[c];
+      r.b1 = a * p.h[pc];
+      const bool neg = (p.neg_mask >> c) & 1;
+      const int64_t dn = pa::neighbour<T, 6>(p, e, sub, neg ? c : c + 6);  // outflow side
+      const int64_t un = pa::neighbour<T, 6>(p, e, sub, neg ? c + 6 : c);  // inflow side
+      const bool own = dn >= 0 || p.bc_mode == GD_BC_INFLOW;
+      r.flags = (own ? 1 : 0) | (neg ? 2 : 0);
+      if (un >= 0) {
+        r.nbn = p.order[un] + 1;
+        r.off = p.dof_offset[un];
+      } else if (p.bc_mode == GD_BC_INFLOW) {
+        r.nbn = N;
+        r.flags |= 8;
+      }
+    }
+    ctab[c] = r;
+  }
   gsync<G>();
 
   // ---- A: pair (2,5) ----
@@ -564,25 +600,12 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
   gsync<G>();
 
   // ---- inflow faces ----
-  // (the component is picked by selects, so the per-component data is never
-  // indexed dynamically, which would put it on the stack)
-  // (faces index the components at run time: they read them from a small
-  // LDS table, so the per-component registers are never indexed dynamically,
+  // (faces index the components at run time: they read them from the LDS
+  // records, so the per-component registers are never indexed dynamically,
   // which would put them on the stack)
-  T* ctab = slot + slot_words<N>() - kCompWords;
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      ctab[4 * k] = cp[k].kt;
-      ctab[4 * k + 1] = cp[k].b0;
-      ctab[4 * k + 2] = cp[k].b1;
-      ctab[4 * k + 3] = T((cp[k].own ? 1 : 0) + (cp[k].neg ? 2 : 0));
-    }
-  }
-  gsync<G>();
   auto pick = [&](int c) -> Comp<T> {
-    const int f = (int)ctab[4 * c + 3];
-    return Comp<T>{ctab[4 * c], ctab[4 * c + 1], ctab[4 * c + 2], (f & 1) != 0, (f & 2) != 0};
+    const CompRec<T> r = ctab[c];
+    return Comp<T>{r.kt, r.b0, r.b1, (r.flags & 1) != 0, (r.flags & 2) != 0};
   };
   if constexpr (ADV) {
     if constexpr (G == 64) {
@@ -591,7 +614,9 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
       for (int c = 0; c < 6; ++c) {
         if (!((act >> c) & 1)) continue;
         const Comp<T> cc = pick(c);
-        if (face_dispatch<T, N>(p, x, e, sub, c, cc, scr, tid)) {
+        const int nbn = ctab[c].nbn, fl = ctab[c].flags;
+        const long long noff = ctab[c].off;
+        if (face_dispatch<T, N>(p, x, c, nbn, noff, (fl & 8) == 0, cc, scr, tid)) {
           face_add_any<T, N>(c, cc, scr, X0, tid);
           gsync<64>();
         }
@@ -609,8 +634,11 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
 #pragma unroll
         for (int k = 0; k < 6; ++k)
           if (((act >> k) & 1) && __builtin_popcount(act & ((1 << k) - 1)) == r0 + w) c = k;
-        const Comp<T> cc = pick(c < 0 ? 0 : c);
-        const bool mine = c >= 0 && face_dispatch<T, N>(p, x, e, sub, c, cc, scr, lane);
+        const int cs = c < 0 ? 0 : c;
+        const Comp<T> cc = pick(cs);
+        const int nbn = ctab[cs].nbn, fl = ctab[cs].flags;
+        const long long noff = ctab[cs].off;
+        const bool mine = c >= 0 && face_dispatch<T, N>(p, x, c, nbn, noff, (fl & 8) == 0, cc, scr, lane);
         __syncthreads();
 #pragma unroll 1
         for (int turn = 0; turn < 4; ++turn) {
