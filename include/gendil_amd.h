@@ -145,17 +145,22 @@ int gd_adv_grad_beta_f64(const gd_plan* plan, double adv_coef, const double* u,
  * vectors, keeps the fastest for later applies of that precision (4 or 8
  * bytes), and reports it.  Uniform plans: the generic LDS-staged kernel (0),
  * the register-resident 3D kernel with 1, 2 or 4 lanes per element, and the
- * Kronecker-factored 3D kernel (16 + lanes per element: 17, 18, 20) where
- * they apply.  p-adaptive plans: the thread-group layouts of the persistent
- * kernel (6D: 0 = one wave per n=3 element, 1 = one block per n=3 element;
- * n=4 elements take a block, n=2 elements a wave).  No GenDiL counterpart:
+ * Kronecker-factored 3D kernel (16 + lanes per element: 17, 18, 20) and its
+ * LDS-staged form (48..55: 48 + 1 outputs staged through LDS, + 2 blocks of
+ * 128 threads, + 4 one term at a time) where they apply.  p-adaptive plans:
+ * the thread-group layouts of the quadrature kernel (0 = one wave per n=3
+ * element, 1 = one block per n=3 element; n=4 elements take a block, n=2
+ * elements a wave) and, in 6D for a field with paired affine components of
+ * one sign each, 2 = the Kronecker-factored kernel.  No GenDiL counterpart:
  * GenDiL fixes its thread layout at compile time (ThreadBlockLayout,
  * threadlayout.hpp:46). */
 int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, double adv_coef,
                      int32_t reps, int32_t* chosen, double* best_ms);
 /* Pin a variant: -1 (default); uniform plans 0 (generic), 1, 2 or 4 lanes
- * (register kernel), 17, 18 or 20 (Kronecker kernel); p-adaptive plans the
- * variant index as reported by gd_plan_autotune. */
+ * (register kernel), 17, 18 or 20 (Kronecker kernel), 48..55 (LDS-staged
+ * Kronecker kernel); p-adaptive plans the variant index as reported by
+ * gd_plan_autotune.  A pinned variant that does not serve a later apply
+ * falls back to the default for that apply. */
 int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes);
 /* The variant an apply with these coefficients would run now (after falling
  * back from a pinned variant that does not serve the plan), in the codes above. */
