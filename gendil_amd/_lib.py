@@ -115,6 +115,7 @@ SIGNATURES = {
     "gd_memcpy_d2h": (C.c_int, [_p, _p, C.c_size_t]),
     "gd_memset": (C.c_int, [_p, C.c_int, C.c_size_t, _p]),
     "gd_stream_sync": (C.c_int, [_p]),
+    "gd_stream_query": (C.c_int, [_p, C.POINTER(C.c_int32)]),
     "gd_device_sync": (C.c_int, []),
     "gd_multidot_f64": (C.c_int, [_i64, _i32, _p, _i64, _p, _p, _p]),
     "gd_multidot_f32": (C.c_int, [_i64, _i32, _p, _i64, _p, _p, _p]),

@@ -592,6 +592,15 @@ int gd_stream_sync(gd_stream stream)
   return GD_OK;
 }
 
+int gd_stream_query(gd_stream stream, int32_t* done)
+{
+  if (!done) return fail(GD_ERR_INVALID, "gd_stream_query: null");
+  const hipError_t e = hipStreamQuery((hipStream_t)stream);
+  *done = e == hipSuccess ? 1 : 0;
+  if (e != hipErrorNotReady) GD_HIP(e);
+  return GD_OK;
+}
+
 int gd_device_sync(void)
 {
   GD_HIP(hipDeviceSynchronize());
@@ -601,7 +610,8 @@ int gd_device_sync(void)
 int gd_host_alloc(void** ptr, size_t bytes)
 {
   if (!ptr) return fail(GD_ERR_INVALID, "gd_host_alloc: null");
-  if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess)
+  // coherent: device stores become visible to host reads without a flush
+  if (hipHostMalloc(ptr, bytes, hipHostMallocCoherent) != hipSuccess)
     return fail(GD_ERR_ALLOC, "gd_host_alloc: hipHostMalloc failed");
   return GD_OK;
 }

@@ -178,6 +178,9 @@ int gd_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int gd_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int gd_memset(void* dst, int value, size_t bytes, gd_stream stream);
 int gd_stream_sync(gd_stream stream);
+/* *done = 1 once every operation queued on the stream has finished, 0 while
+ * some is pending (hipStreamQuery); an error code if the stream failed. */
+int gd_stream_query(gd_stream stream, int32_t* done);
 int gd_device_sync(void);
 
 /* result[j] = sum_i V_j[i] * w[i] for j < k; V_j = V + j*ldv (FP64 accumulate),
@@ -207,26 +210,31 @@ int gd_add_f32_to_f64(int64_t n, double scale, const float* x, double* y, gd_str
  * whose last block also forms the Hessenberg column and its Givens rotation.
  * The workspace (gd_gmres_workspace_bytes(), device memory) holds H, the
  * rotations, g, the basis scales and the reduction partials.
- *   gd_gmres_begin:  g = beta e_0, scale of V_0 = s0.
+ *   gd_gmres_begin:  g = beta e_0, scale of V_0 = s0; starts every cycle
+ *     (it also resets the workspace's pass bookkeeping).
  *   gd_arnoldi_step: V + (k+1)*ldv holds K V_k on entry and the next basis
  *     vector on exit; if resid_out is not null, |g_{k+1}| (the GMRES residual
- *     estimate) is written there by the device (pinned host memory from
- *     gd_host_alloc is allowed).
+ *     estimate) is stored there with a system-scope store (pinned host memory
+ *     from gd_host_alloc is allowed and may be polled) by the first block of
+ *     the next step or update on this workspace, early in that kernel, so no
+ *     kernel's completion waits for the host write.  A 16-byte aligned FP32
+ *     V with ldv a multiple of 4 runs 16-byte accesses.
  *   gd_gmres_update: x += V_{0..k-1} y with y from the k x k triangular
- *     system (gmres.hpp:240-265). */
+ *     system (gmres.hpp:240-265); stores the last step's residual estimate. */
 size_t gd_gmres_workspace_bytes(void);
 int gd_gmres_begin(void* ws, double beta, double s0, gd_stream stream);
 int gd_arnoldi_step_f64(int64_t n, int32_t m, int32_t k, double* V, int64_t ldv, void* ws, double* resid_out,
                         gd_stream stream);
 int gd_arnoldi_step_f32(int64_t n, int32_t m, int32_t k, float* V, int64_t ldv, void* ws, double* resid_out,
                         gd_stream stream);
-int gd_gmres_update_f64(int64_t n, int32_t m, int32_t k, const double* V, int64_t ldv, const void* ws, double* x,
+int gd_gmres_update_f64(int64_t n, int32_t m, int32_t k, const double* V, int64_t ldv, void* ws, double* x,
                         gd_stream stream);
-int gd_gmres_update_f32(int64_t n, int32_t m, int32_t k, const float* V, int64_t ldv, const void* ws, double* x,
+int gd_gmres_update_f32(int64_t n, int32_t m, int32_t k, const float* V, int64_t ldv, void* ws, double* x,
                         gd_stream stream);
 
-/* Pinned host memory and events, so a host loop can read device results
- * without draining the stream. */
+/* Pinned host memory (coherent: a host loop can poll values the device
+ * stores there, e.g. gd_arnoldi_step's resid_out, without draining the
+ * stream) and events. */
 typedef void* gd_event;
 int gd_host_alloc(void** ptr, size_t bytes);
 int gd_host_free(void* ptr);

@@ -163,17 +163,36 @@ inline double dev_dot(int64_t n, const double* a, const double* b, double* scrat
 }
 
 template <typename TV>
-int arnoldi_step(int64_t n, int m, int k, TV* V, void* ws, double* resid, gd_stream st)
+int arnoldi_step(int64_t n, int m, int k, TV* V, int64_t ldv, void* ws, double* resid, gd_stream st)
 {
-  if constexpr (std::is_same_v<TV, double>) return gd_arnoldi_step_f64(n, m, k, V, n, ws, resid, st);
-  else return gd_arnoldi_step_f32(n, m, k, V, n, ws, resid, st);
+  if constexpr (std::is_same_v<TV, double>) return gd_arnoldi_step_f64(n, m, k, V, ldv, ws, resid, st);
+  else return gd_arnoldi_step_f32(n, m, k, V, ldv, ws, resid, st);
 }
 
 template <typename TV>
-int update(int64_t n, int m, int k, const TV* V, const void* ws, double* x, gd_stream st)
+int update(int64_t n, int m, int k, const TV* V, int64_t ldv, void* ws, double* x, gd_stream st)
 {
-  if constexpr (std::is_same_v<TV, double>) return gd_gmres_update_f64(n, m, k, V, n, ws, x, st);
-  else return gd_gmres_update_f32(n, m, k, V, n, ws, x, st);
+  if constexpr (std::is_same_v<TV, double>) return gd_gmres_update_f64(n, m, k, V, ldv, ws, x, st);
+  else return gd_gmres_update_f32(n, m, k, V, ldv, ws, x, st);
+}
+
+// Waits until the device has stored a value (>= 0, or NaN) over the negative
+// marker at *p, checking the stream for errors now and then.
+inline double wait_for_value(const double* p, gd_stream st)
+{
+  for (unsigned spin = 1;; ++spin) {
+    const double v = *static_cast<const volatile double*>(p);
+    if (!(v < 0.0)) return v;
+    if (spin % 4096 == 0) {
+      int32_t done = 0;
+      check(gd_stream_query(st, &done), "gd_stream_query");
+      if (done) {
+        const double w = *static_cast<const volatile double*>(p);
+        if (!(w < 0.0)) return w;
+        throw Error(GD_ERR_HIP, "gmres: the stream finished without storing the residual estimate");
+      }
+    }
+  }
 }
 
 }  // namespace detail
@@ -182,9 +201,11 @@ int update(int64_t n, int m, int k, const TV* V, const void* ws, double* x, gd_s
 //
 // The host never waits inside a cycle for the iteration it has just queued:
 // iteration k is enqueued before the residual estimate of iteration k-1 is
-// read (from pinned memory, after that iteration's event), so the stream
-// always holds work.  When iteration k-1 meets the tolerance, the speculative
-// iteration k is discarded (its column is not used by the update).
+// read (polled in pinned memory, where the device stores it; no event or copy
+// enters the stream), so the stream always holds work.  When iteration k-1
+// meets the tolerance, the speculative iteration k is discarded (its column is
+// not used by the update).  The basis leading dimension is rounded up to 4
+// values so every vector starts 16-byte aligned.
 template <typename TV>
 SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, double* x, const SolveOptions& opt)
 {
@@ -192,11 +213,11 @@ SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, d
   const auto t0 = std::chrono::steady_clock::now();
   const int m = opt.restart;
   const gd_stream st = K.stream;
-  DeviceArray<TV> V((size_t)(m + 1) * n);
+  const int64_t ldv = (n + 3) / 4 * 4;
+  DeviceArray<TV> V((size_t)(m + 1) * ldv);
   DeviceArray<double> r(n), dot(2);
   DeviceArray<unsigned char> ws(gd_gmres_workspace_bytes());
   PinnedArray<double> resid(m);
-  std::vector<Event> done(m);
 
   SolveReport rep;
   const double bnorm = std::sqrt(detail::dev_dot(n, b, b, dot.data(), st));
@@ -227,24 +248,20 @@ SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, d
     const int budget = std::min(m, opt.max_iters - rep.iterations);
     int used = budget;  // columns that enter the update
     for (int k = 0; k < budget; ++k) {
-      TV* vk = V.data() + (size_t)k * n;
-      K.apply(1.0, vk, vk + n);
-      check(detail::arnoldi_step<TV>(n, m, k, V.data(), ws.data(), resid.data() + k, st), "gd_arnoldi_step");
-      done[k].record(st);
+      TV* vk = V.data() + (size_t)k * ldv;
+      resid[k] = -1.0;  // overwritten by the device when iteration k ends
+      K.apply(1.0, vk, vk + ldv);
+      check(detail::arnoldi_step<TV>(n, m, k, V.data(), ldv, ws.data(), resid.data() + k, st), "gd_arnoldi_step");
       if (k >= 1) {
-        done[k - 1].sync();
-        const double e = resid[k - 1];
+        const double e = detail::wait_for_value(resid.data() + k - 1, st);
         if (e <= target) {  // converged (or happy breakdown) at iteration k-1
           used = k;
           break;
         }
       }
     }
-    if (used == budget && budget > 0) {
-      done[budget - 1].sync();
-    }
     rep.iterations += used;
-    check(detail::update<TV>(n, m, used, V.data(), ws.data(), x, st), "gd_gmres_update");
+    check(detail::update<TV>(n, m, used, V.data(), ldv, ws.data(), x, st), "gd_gmres_update");
   }
   check(gd_stream_sync(st), "sync");
   rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

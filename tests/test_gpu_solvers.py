@@ -125,6 +125,10 @@ def test_arnoldi_basis_is_orthonormal(dtype):
         else:
             L.check(lib.gd_apply_f32(K.plan.handle, 1.0, K.adv_coef, V[k].data_ptr(), V[k + 1].data_ptr(), st), "f32")
         L.check(step(n, m, k, V.data_ptr(), n, ws.data_ptr(), resid[k:].data_ptr(), st), "step")
+    # the update stores the last step's residual estimate
+    upd = lib.gd_gmres_update_f64 if dtype == torch.float64 else lib.gd_gmres_update_f32
+    xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    L.check(upd(n, m, m, V.data_ptr(), n, ws.data_ptr(), xs.data_ptr(), st), "update")
     torch.cuda.synchronize()
     Vd = V.double()
     gram = Vd @ Vd.T
@@ -133,4 +137,59 @@ def test_arnoldi_basis_is_orthonormal(dtype):
     print(f"{dtype}: max |V^T V - I| = {err:.3e}, residual estimates {resid.tolist()[:4]} ...")
     assert err <= tol
     r = resid.cpu().numpy()
+    assert np.all(r > 0)
     assert np.all(np.diff(r) <= 1e-12 * r[0])  # GMRES residual estimates never increase
+    # x = V_m y minimises ||b - K x||: its true residual is the last estimate
+    Kx = torch.empty_like(xs)
+    K(xs, Kx)
+    true = float((b - Kx).norm())
+    tolr = 1e-9 if dtype == torch.float64 else 1e-4
+    assert abs(true - r[-1]) <= tolr * float(b.norm()), (true, r[-1])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("n,pad", [(1001, 0), (1001, 3), (4096, 0), (4096, 1)])
+def test_arnoldi_layouts(dtype, n, pad):
+    # Arnoldi on a dense random operator with every basis layout: packed
+    # 16-byte accesses (aligned rows), a length that leaves a tail after the
+    # last pack, and leading dimensions that force one value per access.
+    from gendil_amd import _lib as L
+
+    g = torch.Generator(device="cuda").manual_seed(7 + n + pad)
+    A = torch.randn((n, n), generator=g, device="cuda", dtype=torch.float64) / n ** 0.5
+    A += 2.0 * torch.eye(n, device="cuda", dtype=torch.float64)
+    Ad = A.to(dtype)
+    m = 10
+    ldv = n + pad
+    lib = L.lib()
+    V = torch.zeros((m + 1, ldv), dtype=dtype, device="cuda")
+    b = torch.randn(n, generator=g, device="cuda", dtype=torch.float64)
+    V[0, :n] = (b / b.norm()).to(dtype)
+    ws = torch.zeros(lib.gd_gmres_workspace_bytes(), dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    L.check(lib.gd_gmres_begin(ws.data_ptr(), float(b.norm()), 1.0, st), "begin")
+    resid = torch.zeros(m, dtype=torch.float64, device="cuda")
+    step = lib.gd_arnoldi_step_f64 if dtype == torch.float64 else lib.gd_arnoldi_step_f32
+    for k in range(m):
+        V[k + 1, :n] = Ad @ V[k, :n]
+        L.check(step(n, m, k, V.data_ptr(), ldv, ws.data_ptr(), resid[k:].data_ptr(), st), "step")
+    upd = lib.gd_gmres_update_f64 if dtype == torch.float64 else lib.gd_gmres_update_f32
+    xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    L.check(upd(n, m, m, V.data_ptr(), ldv, ws.data_ptr(), xs.data_ptr(), st), "update")
+    torch.cuda.synchronize()
+    Vd = V[:, :n].double()
+    err = float((Vd @ Vd.T - torch.eye(m + 1, dtype=torch.float64, device="cuda")).abs().max())
+    assert err <= (1e-13 if dtype == torch.float64 else 1e-6), err
+    if pad:
+        assert float(V[:, n:].abs().max()) == 0.0  # padding untouched
+    r = resid.cpu().numpy()
+    assert np.all(r > 0) and np.all(np.diff(r) <= 1e-12 * r[0])
+    # residual estimate of step m equals the true least-squares residual over
+    # the Krylov space, min_y ||b - A V_m y|| (FP64 dense solve)
+    if dtype == torch.float64:
+        Km = A @ Vd[:m].T
+        y = torch.linalg.lstsq(Km.cpu(), b.cpu().unsqueeze(1)).solution.to("cuda").squeeze(1)
+        true = float((b - Km @ y).norm())
+        assert abs(r[-1] - true) <= 1e-10 * float(b.norm()), (r[-1], true)
+        # and the update's x = V_m y attains it
+        assert abs(float((b - A @ xs).norm()) - true) <= 1e-10 * float(b.norm())
