@@ -50,7 +50,9 @@ constexpr int ipw(int b, int e) { return e == 0 ? 1 : b * ipw(b, e - 1); }
 // are XOR-ed with (i4, i3, i3) in their three 2-bit fields: the 64 lanes of a
 // wave, which in the three pair phases run over (i0,i1,i3), (i0,i2,i3) and
 // (i1,i2,i4) with the rest fixed, then hit 64 different banks in every phase
-// (with the plain layout phase (0,3) is 4-way conflicted).  A bijection.
+// (with the plain layout phase (0,3) is 4-way conflicted).  A bijection, and
+// linear over XOR: swz(a ^ b) = swz(a) ^ swz(b), so an index assembled from
+// disjoint bit fields swizzles field by field (compile-time fields fold).
 template <int N>
 __device__ __forceinline__ int swz(int idx)
 {
@@ -69,6 +71,10 @@ __device__ __forceinline__ int swz(int idx)
 // a resident block per CU).
 template <bool PAD>
 constexpr int scr_words() { return PAD ? 1280 : 1024; }
+
+// global loads of a face slab in flight per lane (8 measured fastest; all 16
+// of an n = 4 slab at once spill registers)
+constexpr int SLAB_CHUNK = 8;
 template <bool PAD>
 constexpr int sstride(int k) { return k == 0 ? 1 : (PAD ? 5 : 4) * ipw(4, k - 1); }
 
@@ -243,48 +249,59 @@ __device__ __forceinline__ void add_g(T (&Y)[N][N], const T (&S)[N][N], const Co
   }
 }
 
-// offset of slice t of pair (a, a+3): the other four axes, ascending, from t
-template <int N>
-__device__ __forceinline__ int slice_base(int a, int t)
+// offset of slice t of pair (A, A+3): the other four axes, ascending, from t
+template <int N, int A>
+__device__ __forceinline__ int slice_base(int t)
 {
   int base = 0, r = t;
 #pragma unroll
   for (int d = 0; d < 6; ++d)
-    if (d != a && d != a + 3) {
+    if (d != A && d != A + 3) {
       base += (r % N) * ipw(N, d);
       r /= N;
     }
   return base;
 }
 
-template <typename T, int N, int SPT>
-__device__ __forceinline__ void read_slices(const T* X, int a, int tid, int G, T (&S)[SPT][N][N])
+// LDS position of entry (i, j) of the slice whose swizzled base is sb.  For
+// N = 4 the slice's axes are zero bit fields of the base, so b + i s_A + j s_A3
+// = b ^ (i s_A) ^ (j s_A3) and the swizzle of the constant part folds: one XOR
+// per entry.  Other N are not swizzled and add.
+template <int N, int A>
+__device__ __forceinline__ int slice_pos(int sb, int i, int j)
+{
+  if constexpr (N == 4) return sb ^ swz<N>(i * ipw(N, A) + j * ipw(N, A + 3));
+  else return sb + i * ipw(N, A) + j * ipw(N, A + 3);
+}
+
+template <typename T, int N, int SPT, int A>
+__device__ __forceinline__ void read_slices(const T* X, int tid, int G, T (&S)[SPT][N][N])
 {
   constexpr int N4 = ipw(N, 4);
 #pragma unroll
   for (int s = 0; s < SPT; ++s) {
     const int t = min(tid + s * G, N4 - 1);
-    const int b = slice_base<N>(a, t);
+    const int sb = swz<N>(slice_base<N, A>(t));
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) S[s][i][j] = X[swz<N>(b + i * ipw(N, a) + j * ipw(N, a + 3))];
+      for (int j = 0; j < N; ++j) S[s][i][j] = X[slice_pos<N, A>(sb, i, j)];
   }
 }
 
-template <typename T, int N, int SPT>
-__device__ __forceinline__ void write_slices(T* X, int a, int tid, int G, const T (&S)[SPT][N][N])
+template <typename T, int N, int SPT, int A>
+__device__ __forceinline__ void write_slices(T* X, int tid, int G, const T (&S)[SPT][N][N])
 {
   constexpr int N4 = ipw(N, 4);
 #pragma unroll
   for (int s = 0; s < SPT; ++s) {
     const int t = tid + s * G;
     if (t < N4) {
-      const int b = slice_base<N>(a, t);
+      const int sb = swz<N>(slice_base<N, A>(t));
 #pragma unroll
       for (int i = 0; i < N; ++i)
 #pragma unroll
-        for (int j = 0; j < N; ++j) X[swz<N>(b + i * ipw(N, a) + j * ipw(N, a + 3))] = S[s][i][j];
+        for (int j = 0; j < N; ++j) X[slice_pos<N, A>(sb, i, j)] = S[s][i][j];
     }
   }
 }
@@ -308,19 +325,39 @@ __device__ __forceinline__ void face_convert(const T* __restrict__ xnb, bool dat
   constexpr int PC = (C + 3) % 6;
   // the neighbour's face-node slab (or the constant inflow data)
   {
+    // the slab's loads issued SLAB_CHUNK at a time per lane, then stored to
+    // the scratch (one load per loop trip left each load's latency exposed)
     constexpr int NS = ipw(NB, 5);
+    constexpr int LPS = (NS + WL - 1) / WL;
+    constexpr int CH = LPS < SLAB_CHUNK ? LPS : SLAB_CHUNK;  // loads in flight per lane
     const int ns = cc.neg ? 0 : NB - 1;  // neighbour node at the shared face
+    const T* __restrict__ xs = xnb + (data ? ns * ipw(NB, C) : 0);
 #pragma unroll 1
-    for (int v = lane; v < NS; v += WL) {
-      int r = v, src = 0, dst = 0;
+    for (int c0 = 0; c0 < LPS; c0 += CH) {
+      T val[CH];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        const int dg = r % NB;
-        r /= NB;
-        src += dg * ipw(NB, FA::ax(k));
-        dst += dg * sstride<false>(k);
+      for (int u = 0; u < CH; ++u) {
+        int r = min(lane + (c0 + u) * WL, NS - 1), src = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          src += (r % NB) * ipw(NB, FA::ax(k));
+          r /= NB;
+        }
+        val[u] = data ? xs[src] : g;
       }
-      scr[dst] = data ? xnb[src + ns * ipw(NB, C)] : g;
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int v = lane + (c0 + u) * WL;
+        if (c0 + u < LPS && (NS % WL == 0 || v < NS)) {
+          int r = v, dst = 0;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            dst += (r % NB) * sstride<false>(k);
+            r /= NB;
+          }
+          scr[dst] = val[u];
+        }
+      }
     }
   }
   gsync<64>();
@@ -370,18 +407,30 @@ __device__ __forceinline__ void face_add(const Comp<T>& cc, const T* scr, T* Y, 
   constexpr int WL = 64;
   constexpr int NS = ipw(N, 5);
   using FA = FaceAxes<C>;
+  constexpr int LPS = (NS + WL - 1) / WL;
+  constexpr int CH = LPS < 4 ? LPS : 4;  // entries in flight per lane
   const int ni = cc.neg ? N - 1 : 0;
 #pragma unroll 1
-  for (int v = lane; v < NS; v += WL) {
-    int r = v, dst = 0, src = 0;
+  for (int c0 = 0; c0 < LPS; c0 += CH) {
+    int dst[CH];
+    T sv[CH], yv[CH];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int dg = r % N;
-      r /= N;
-      dst += dg * ipw(N, FA::ax(k));
-      src += dg * sstride<false>(k);
+    for (int u = 0; u < CH; ++u) {
+      int r = min(lane + (c0 + u) * WL, NS - 1), d = 0, src = 0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int dg = r % N;
+        r /= N;
+        d += dg * ipw(N, FA::ax(k));
+        src += dg * sstride<false>(k);
+      }
+      dst[u] = swz<N>(d + ni * ipw(N, C));
+      sv[u] = scr[src];
+      yv[u] = Y[dst[u]];
     }
-    Y[swz<N>(dst + ni * ipw(N, C))] += scr[src];
+#pragma unroll
+    for (int u = 0; u < CH; ++u)
+      if (c0 + u < LPS && (NS % WL == 0 || lane + (c0 + u) * WL < NS)) Y[dst[u]] = yv[u] + sv[u];
   }
 }
 
@@ -469,8 +518,15 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
   const int64_t off = p.dof_offset[e];
   int sub[6];
   pa::decode<T, 6>(p, e, sub);
-#pragma unroll 1
-  for (int i = tid; i < N6; i += G) X0[swz<N>(i)] = x[off + i];
+  constexpr int LPT = (N6 + G - 1) / G;  // entries per thread
+  {
+    T v[LPT];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) v[k] = x[off + min(tid + k * G, N6 - 1)];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k)
+      if (N6 % G == 0 || tid + k * G < N6) X0[swz<N>(tid + k * G)] = v[k];
+  }
 
   // per-element component data
   Comp<T> cp[6];
@@ -525,7 +581,7 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
 
   // ---- A: pair (2,5) ----
   T S[SPT][N][N], U1[SPT][N][N];
-  read_slices<T, N, SPT>(X0, 2, tid, G, S);
+  read_slices<T, N, SPT, 2>(X0, tid, G, S);
   gsync<G>();
 #pragma unroll
   for (int s = 0; s < SPT; ++s) {
@@ -545,13 +601,13 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
 #pragma unroll
       for (int j = 0; j < N; ++j) S[s][i][j] = P[i][j];
   }
-  write_slices<T, N, SPT>(X0, 2, tid, G, S);   // P
-  write_slices<T, N, SPT>(X1, 2, tid, G, U1);  // Q
+  write_slices<T, N, SPT, 2>(X0, tid, G, S);   // P
+  write_slices<T, N, SPT, 2>(X1, tid, G, U1);  // Q
   gsync<G>();
 
   // ---- B: pair (1,4) ----
-  read_slices<T, N, SPT>(X0, 1, tid, G, S);   // P
-  read_slices<T, N, SPT>(X1, 1, tid, G, U1);  // Q
+  read_slices<T, N, SPT, 1>(X0, tid, G, S);   // P
+  read_slices<T, N, SPT, 1>(X1, tid, G, U1);  // Q
   gsync<G>();
 #pragma unroll
   for (int s = 0; s < SPT; ++s) {
@@ -571,13 +627,13 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
         U1[s][i][j] = Wv[i][j];
       }
   }
-  write_slices<T, N, SPT>(X0, 1, tid, G, S);   // R
-  write_slices<T, N, SPT>(X1, 1, tid, G, U1);  // W
+  write_slices<T, N, SPT, 1>(X0, tid, G, S);   // R
+  write_slices<T, N, SPT, 1>(X1, tid, G, U1);  // W
   gsync<G>();
 
   // ---- C: pair (0,3) ----
-  read_slices<T, N, SPT>(X0, 0, tid, G, S);   // R
-  read_slices<T, N, SPT>(X1, 0, tid, G, U1);  // W
+  read_slices<T, N, SPT, 0>(X0, tid, G, S);   // R
+  read_slices<T, N, SPT, 0>(X1, tid, G, U1);  // W
   gsync<G>();
 #pragma unroll
   for (int s = 0; s < SPT; ++s) {
@@ -596,7 +652,7 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
 #pragma unroll
       for (int j = 0; j < N; ++j) S[s][i][j] = Yv[i][j];
   }
-  write_slices<T, N, SPT>(X0, 0, tid, G, S);  // Y without the inflow faces
+  write_slices<T, N, SPT, 0>(X0, tid, G, S);  // Y without the inflow faces
   gsync<G>();
 
   // ---- inflow faces ----
@@ -648,8 +704,14 @@ __device__ __forceinline__ void element(const PParams<T>& p, T mass_coef, T adv_
       }
     }
   }
-#pragma unroll 1
-  for (int i = tid; i < N6; i += G) y[off + i] = X0[swz<N>(i)];
+  {
+    T v[LPT];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) v[k] = X0[swz<N>(min(tid + k * G, N6 - 1))];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k)
+      if (N6 % G == 0 || tid + k * G < N6) y[off + tid + k * G] = v[k];
+  }
 }
 
 // One launch: it0 blocks of one n = 4 element each, then it1 blocks of four
