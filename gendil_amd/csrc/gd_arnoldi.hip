@@ -38,16 +38,16 @@
 #include <cstdint>
 
 #include "gd_internal.hpp"
+#include "gd_reduce.hpp"
 
 namespace gd {
 namespace arnoldi {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
+using red::kFinThreads;
+using red::kMaxBlocks;
+using red::kThreads;
 constexpr int kMaxM = 32;           // restart length
-constexpr int kMaxBlocks = 1024;
 constexpr int kStride = kMaxM + 2;  // partial columns per block: k+1 dots + norm
-constexpr int kFinThreads = 1024;   // finishing kernels: one block's partials per thread
 static_assert(kFinThreads == kMaxBlocks, "one block's partials per finishing thread");
 
 struct Workspace {
@@ -82,99 +82,21 @@ __device__ __forceinline__ void st(T* p, const Pack<T, W>& v)
   *reinterpret_cast<Pack<T, W>*>(p) = v;
 }
 
-inline int grid_for(int64_t n, int w)
-{
-  const int64_t per = (int64_t)kThreads * 4 * w;
-  const int64_t b = (n + per - 1) / per;
-  return (int)(b < 1 ? 1 : (b > kMaxBlocks ? kMaxBlocks : b));
-}
+using red::grid_for;
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// out[j] = sum over the block's NT threads of acc[j] (waves in order); out is
-// visible to every thread on return.
-template <int NT, int K>
-__device__ __forceinline__ void block_sum(const double (&acc)[K], double* out)
-{
-  constexpr int NW = NT / 64;
-  __shared__ double red[NW][K];
-  const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    const double v = wave_sum(acc[j]);
-    if (lane == 0) red[wv][j] = v;
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < K; j += NT) {
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) s += red[w][j];
-    out[j] = s;
-  }
-  __syncthreads();
-}
-
-// The block's column sums as its partial row.
 template <int K>
 __device__ __forceinline__ void store_block_partials(const double (&acc)[K], Workspace* ws)
 {
-  __shared__ double row[K];
-  block_sum<kThreads, K>(acc, row);
-  for (int j = threadIdx.x; j < K; j += kThreads) ws->partial[j][blockIdx.x] = row[j];
+  red::store_block_partials<K>(acc, ws->partial);
 }
 
-// col[j] = sum over the nblk blocks' partials of column j, j < K: 32 threads
-// per column, each summing every 32nd block (consecutive threads on
-// consecutive blocks, all loads in flight together), then a 5-step shuffle
-// sum over the 32 threads.  A fixed order for a given grid; visible to every
-// thread on return.
 template <int K>
 __device__ __forceinline__ void sum_partials(const Workspace* ws, int nblk, double* col)
 {
-  constexpr int TPC = 32;                  // threads per column
-  constexpr int CPP = kFinThreads / TPC;   // columns per round
-  constexpr int PER = kMaxBlocks / TPC;    // blocks per thread
-  const int lane = threadIdx.x % TPC, c0 = threadIdx.x / TPC;
-#pragma unroll
-  for (int r = 0; r < (K + CPP - 1) / CPP; ++r) {
-    const int j = c0 + r * CPP;
-    if (j < K) {  // uniform over each 32-thread group; the shuffles stay inside it
-      const double* cp = ws->partial[j];
-      double v[PER];
-#pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int bb = lane + TPC * i;
-        const double x = cp[bb < nblk ? bb : nblk - 1];
-        v[i] = bb < nblk ? x : 0.0;
-      }
-      double sum = 0.0;
-#pragma unroll
-      for (int i = 0; i < PER; ++i) sum += v[i];
-#pragma unroll
-      for (int off = TPC / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, TPC);
-      if (lane == 0) col[j] = sum;
-    }
-  }
-  __syncthreads();
+  red::sum_partials<K>(ws->partial, nblk, col);
 }
 
-// First block, first thread: hand the previous step's residual estimate to
-// the caller's pointer (system scope: the pointer may be pinned host memory).
-__device__ __forceinline__ void publish_pending(Workspace* ws)
-{
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double* p = ws->pending;
-    if (p) {
-      __hip_atomic_store(p, ws->pending_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      ws->pending = nullptr;
-    }
-  }
-}
+__device__ __forceinline__ void publish_pending(Workspace* ws) { red::publish_pending(&ws->pending, &ws->pending_value); }
 
 // The basis passes take the number of stored vectors KK as a template
 // parameter: every load of a pass is then unconditional and independent, so

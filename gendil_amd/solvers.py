@@ -1,5 +1,6 @@
 """Device-resident Krylov solves over the C-ABI (gd_gmres_f64 / gd_gmres_mixed /
-gd_cg_f64, implemented by the header-only solvers of include/gendil_amd/krylov.hpp).
+gd_cg_f64 / gd_bicgstab_f64, implemented by the header-only solvers of
+include/gendil_amd/krylov.hpp).
 
 They solve K x = b for K = mass_coef * M + adv_coef * A of one operator plan,
 with GenDiL's GMRES_no_alloc contract (gmres.hpp:74-288): x holds x0 on
@@ -50,5 +51,13 @@ def gmres_mixed(op, b, x, restart=30, max_iters=10000, rel_tol=1e-10, inner_rel_
 
 
 def cg(op, b, x, max_iters=10000, rel_tol=1e-10):
-    """Unpreconditioned CG (conjugategradient.hpp:217) for SPD operators."""
+    """Unpreconditioned CG (conjugategradient.hpp:217) for SPD operators,
+    device-resident (no dot product synchronises the stream)."""
     return _run(L.lib().gd_cg_f64, op, b, x, 30, max_iters, rel_tol, 1, 0.0)
+
+
+def bicgstab(op, b, x, max_iters=10000, rel_tol=1e-10):
+    """Unpreconditioned BiCGSTAB (BiCGSTAB_no_alloc, bicgstab.hpp:44-157),
+    device-resident; stops at ||r|| / ||b|| < rel_tol (the recurrence
+    residual, as the reference does) and reports the true residual."""
+    return _run(L.lib().gd_bicgstab_f64, op, b, x, 30, max_iters, rel_tol, 1, 0.0)

@@ -232,6 +232,36 @@ int gd_gmres_update_f64(int64_t n, int32_t m, int32_t k, const double* V, int64_
 int gd_gmres_update_f32(int64_t n, int32_t m, int32_t k, const float* V, int64_t ldv, void* ws, double* x,
                         gd_stream stream);
 
+/* Device-resident CG and BiCGSTAB iterations (FP64).  The recurrence
+ * scalars, the convergence test and the breakdown guards live on the device
+ * (workspace of gd_ksolve_workspace_bytes(), device memory), so a host loop
+ * issues iterations without waiting for any dot product; once the iteration
+ * converges or stops, every later call on the workspace does nothing.  The
+ * relative residual ||r|| / ||b|| of each iteration goes to resid_out (NaN
+ * once stopped without converging) from the first block of the next kernel
+ * on the workspace.  gd_ksolve_status copies the state (0 running,
+ * 1 converged, 2 stopped), the completed iterations and the residual to the
+ * host (synchronous).
+ *   CG (conjugategradient.hpp:125-235): p = r on entry; per iteration the
+ *     caller forms Ap = K p, then gd_cg_step_f64 does alpha, x, r, the test,
+ *     beta and p.
+ *   BiCGSTAB (bicgstab.hpp:44-157): rhat = r on entry; per iteration
+ *     gd_bicgstab_p_f64 (p), v = K p, gd_bicgstab_s_f64 (alpha, s, the
+ *     early-exit test), t = K s, gd_bicgstab_xr_f64 (omega, x, r, the test,
+ *     rho); the reference's breakdown guards (|.| < 1e-30) and fallbacks. */
+size_t gd_ksolve_workspace_bytes(void);
+int gd_cg_begin_f64(int64_t n, const double* r, double bnorm, double rel_tol, void* ws, gd_stream stream);
+int gd_cg_step_f64(int64_t n, double* x, double* r, double* p, const double* ap, void* ws, double* resid_out,
+                   gd_stream stream);
+int gd_bicgstab_begin_f64(int64_t n, const double* rhat, const double* r, double bnorm, double rel_tol, void* ws,
+                          gd_stream stream);
+int gd_bicgstab_p_f64(int64_t n, const double* r, double* p, const double* v, void* ws, gd_stream stream);
+int gd_bicgstab_s_f64(int64_t n, const double* r, const double* rhat, const double* v, double* s, void* ws,
+                      gd_stream stream);
+int gd_bicgstab_xr_f64(int64_t n, double* x, double* r, const double* rhat, const double* p, const double* s,
+                       const double* t, void* ws, double* resid_out, gd_stream stream);
+int gd_ksolve_status(const void* ws, gd_stream stream, int32_t* state, int32_t* iterations, double* rel_residual);
+
 /* Pinned host memory (coherent: a host loop can poll values the device
  * stores there, e.g. gd_arnoldi_step's resid_out, without draining the
  * stream) and events. */
@@ -272,10 +302,19 @@ int gd_gmres_f64(const gd_plan* plan, double mass_coef, double adv_coef, const d
 int gd_gmres_mixed(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
                    double* x, const gd_solve_params* params, gd_solve_result* result,
                    gd_stream stream);
-/* Unpreconditioned CG (conjugategradient.hpp:217) for SPD operators (mass). */
+/* Unpreconditioned CG (conjugategradient.hpp:217) for SPD operators (mass),
+ * device-resident (gd_cg_step_f64).  Stops at ||r|| / ||b|| <= rel_tol; the
+ * result reports the true residual. */
 int gd_cg_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
               double* x, const gd_solve_params* params, gd_solve_result* result,
               gd_stream stream);
+/* Unpreconditioned BiCGSTAB (BiCGSTAB_no_alloc, bicgstab.hpp:44-157),
+ * device-resident (gd_bicgstab_*_f64): the reference's stopping rule
+ * (||r|| / ||b|| < rel_tol), early exit, fallback and breakdown guards.
+ * x holds x0; the result reports the true residual. */
+int gd_bicgstab_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
+                    double* x, const gd_solve_params* params, gd_solve_result* result,
+                    gd_stream stream);
 
 #ifdef __cplusplus
 }

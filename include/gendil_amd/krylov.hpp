@@ -268,44 +268,107 @@ SolveReport gmres_solve(const CombinedOperator& K, int64_t n, const double* b, d
   return rep;
 }
 
-// Unpreconditioned CG (conjugategradient.hpp:217) for SPD K, FP64.
+namespace detail {
+
+// true relative residual ||b - K x|| / ||b|| (synchronous)
+inline double true_residual(const CombinedOperator& K, int64_t n, const double* b, const double* x, double* r,
+                            double* scratch, double bnorm)
+{
+  K.apply(1.0, x, r);
+  check(gd_axpby_f64(n, 1.0, b, -1.0, r, K.stream), "gd_axpby_f64");
+  return std::sqrt(dev_dot(n, r, r, scratch, K.stream)) / bnorm;
+}
+
+// The device-resident iterations: the host issues iteration k, then waits for
+// the residual estimate of iteration k-1 (so the stream always holds the next
+// iteration); the device itself stops updating once converged or stopped.
+template <typename Issue>
+void drive(int max_iters, double rel_tol, bool strict, double* resid, gd_stream st, Issue&& issue)
+{
+  for (int k = 0; k < max_iters; ++k) {
+    resid[k % 2] = -1.0;
+    issue(k, resid + k % 2);
+    if (k >= 1) {
+      const double e = wait_for_value(resid + (k - 1) % 2, st);
+      if (std::isnan(e) || (strict ? e < rel_tol : e <= rel_tol)) break;
+    }
+  }
+}
+
+}  // namespace detail
+
+// Unpreconditioned CG (conjugategradient.hpp:217) for SPD K, FP64,
+// device-resident (gd_cg_step_f64): no dot product synchronises the stream.
 inline SolveReport cg_solve(const CombinedOperator& K, int64_t n, const double* b, double* x,
                             const SolveOptions& opt)
 {
   const auto t0 = std::chrono::steady_clock::now();
   const gd_stream st = K.stream;
   DeviceArray<double> r(n), p(n), Ap(n), tmp(4);
+  DeviceArray<unsigned char> ws(gd_ksolve_workspace_bytes());
+  PinnedArray<double> resid(2);
   SolveReport rep;
   const double bnorm = std::sqrt(detail::dev_dot(n, b, b, tmp.data(), st));
-  K.apply(1.0, x, r.data());
-  check(gd_axpby_f64(n, 1.0, b, -1.0, r.data(), st), "axpby");
-  check(gd_axpby_f64(n, 1.0, r.data(), 0.0, p.data(), st), "copy");
-  double rr = detail::dev_dot(n, r.data(), r.data(), tmp.data(), st);
-  if (bnorm == 0.0) {
+  if (bnorm == 0.0) {  // zero right-hand side: x = 0
+    check(gd_axpby_f64(n, 0.0, b, 0.0, x, st), "gd_axpby_f64");
+    check(gd_stream_sync(st), "sync");
     rep.success = true;
     return rep;
   }
-  while (rep.iterations < opt.max_iters) {
-    rep.final_rel_residual = std::sqrt(rr) / bnorm;
-    if (rep.final_rel_residual <= opt.rel_tol) {
-      rep.success = true;
-      break;
-    }
-    ++rep.iterations;
-    K.apply(1.0, p.data(), Ap.data());
-    const double pAp = detail::dev_dot(n, p.data(), Ap.data(), tmp.data(), st);
-    const double alpha = rr / pAp;
-    check(gd_axpby_f64(n, alpha, p.data(), 1.0, x, st), "axpby");
-    check(gd_axpby_f64(n, -alpha, Ap.data(), 1.0, r.data(), st), "axpby");
-    const double rr_new = detail::dev_dot(n, r.data(), r.data(), tmp.data(), st);
-    check(gd_axpby_f64(n, 1.0, r.data(), rr_new / rr, p.data(), st), "axpby");
-    rr = rr_new;
-  }
-  // report the true residual
   K.apply(1.0, x, r.data());
   check(gd_axpby_f64(n, 1.0, b, -1.0, r.data(), st), "axpby");
-  rep.final_rel_residual = std::sqrt(detail::dev_dot(n, r.data(), r.data(), tmp.data(), st)) / bnorm;
-  rep.success = rep.final_rel_residual <= opt.rel_tol * 10.0;
+  check(gd_axpby_f64(n, 1.0, r.data(), 0.0, p.data(), st), "copy");
+  check(gd_cg_begin_f64(n, r.data(), bnorm, opt.rel_tol, ws.data(), st), "gd_cg_begin_f64");
+  detail::drive(opt.max_iters, opt.rel_tol, false, resid.data(), st, [&](int, double* out) {
+    K.apply(1.0, p.data(), Ap.data());
+    check(gd_cg_step_f64(n, x, r.data(), p.data(), Ap.data(), ws.data(), out, st), "gd_cg_step_f64");
+  });
+  int32_t state = 0, iters = 0;
+  double rel = 0.0;
+  check(gd_ksolve_status(ws.data(), st, &state, &iters, &rel), "gd_ksolve_status");
+  rep.iterations = iters;
+  rep.success = state == 1;
+  rep.final_rel_residual = detail::true_residual(K, n, b, x, r.data(), tmp.data(), bnorm);
+  rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rep;
+}
+
+// Unpreconditioned BiCGSTAB (BiCGSTAB_no_alloc, bicgstab.hpp:44-157), FP64,
+// device-resident (gd_bicgstab_*_f64).
+inline SolveReport bicgstab_solve(const CombinedOperator& K, int64_t n, const double* b, double* x,
+                                  const SolveOptions& opt)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  const gd_stream st = K.stream;
+  DeviceArray<double> r(n), rhat(n), p(n), v(n), s(n), t(n), tmp(4);
+  DeviceArray<unsigned char> ws(gd_ksolve_workspace_bytes());
+  PinnedArray<double> resid(2);
+  SolveReport rep;
+  const double bnorm = std::sqrt(detail::dev_dot(n, b, b, tmp.data(), st));
+  if (bnorm == 0.0) {  // zero right-hand side: x = 0 (bicgstab.hpp:74-78)
+    check(gd_axpby_f64(n, 0.0, b, 0.0, x, st), "gd_axpby_f64");
+    check(gd_stream_sync(st), "sync");
+    rep.success = true;
+    return rep;
+  }
+  K.apply(1.0, x, r.data());
+  check(gd_axpby_f64(n, 1.0, b, -1.0, r.data(), st), "axpby");
+  check(gd_axpby_f64(n, 1.0, r.data(), 0.0, rhat.data(), st), "copy");
+  check(gd_bicgstab_begin_f64(n, rhat.data(), r.data(), bnorm, opt.rel_tol, ws.data(), st), "gd_bicgstab_begin_f64");
+  detail::drive(opt.max_iters, opt.rel_tol, true, resid.data(), st, [&](int, double* out) {
+    check(gd_bicgstab_p_f64(n, r.data(), p.data(), v.data(), ws.data(), st), "gd_bicgstab_p_f64");
+    K.apply(1.0, p.data(), v.data());
+    check(gd_bicgstab_s_f64(n, r.data(), rhat.data(), v.data(), s.data(), ws.data(), st), "gd_bicgstab_s_f64");
+    K.apply(1.0, s.data(), t.data());
+    check(gd_bicgstab_xr_f64(n, x, r.data(), rhat.data(), p.data(), s.data(), t.data(), ws.data(), out, st),
+          "gd_bicgstab_xr_f64");
+  });
+  int32_t state = 0, iters = 0;
+  double rel = 0.0;
+  check(gd_ksolve_status(ws.data(), st, &state, &iters, &rel), "gd_ksolve_status");
+  rep.iterations = iters;
+  rep.success = state == 1;
+  rep.final_rel_residual = detail::true_residual(K, n, b, x, r.data(), tmp.data(), bnorm);
   rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return rep;
 }

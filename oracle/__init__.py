@@ -11,7 +11,9 @@ import this package, and only as the checker: the product path never calls it.
 Parity status: gdo_apply pinned by golden vectors from _ref/gendil_ref
 (tests/golden); gdo_apply_transpose pinned by dense transposes and the
 dot-product identity; gdo_grad_beta pinned by central finite differences;
-the p-adaptive apply pinned by its uniform special case and by conservation.
+the p-adaptive apply pinned by its uniform special case and by conservation;
+bicgstab pinned by GenDiL's own BiCGSTAB_no_alloc run through _ref/gendil_ref
+(tests/golden/bicgstab3d_c2_p3q4_bc0.npz) and by dense solves.
 """
 from __future__ import annotations
 
@@ -243,3 +245,75 @@ def ref_bench(op, D, P, Q, basis, n, periodic, coef, bc, seed, pairs, env=None, 
     if res.returncode != 0:
         raise RuntimeError(f"gendil_ref bench failed: {res.stderr}")
     return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def ref_bicgstab(D, P, Q, basis, n, coef, bc, seed, path):
+    """GenDiL's BiCGSTAB_no_alloc (bicgstab.hpp:44) on M + 0.05 A, b =
+    FillRandom(seed), x0 = 0, tol 1e-10 (ref_harness.cpp, solve_bicgstab).
+    Returns (b, x, iterations, success, relative residual)."""
+    run_ref(["bicgstab", "madv", D, P, Q, basis, n, 0, coef, bc, seed, path])
+    d = np.fromfile(path, dtype=np.float64)
+    nd = int(d[:1].view(np.int64)[0])
+    b = d[1 : 1 + nd].copy()
+    x = d[1 + nd : 1 + 2 * nd].copy()
+    it, ok, rel = d[1 + 2 * nd : 4 + 2 * nd]
+    return b, x, int(it), bool(ok), float(rel)
+
+
+def bicgstab(apply, b, x, max_iter, tol):
+    """Unpreconditioned BiCGSTAB restated from BiCGSTAB_no_alloc
+    (bicgstab.hpp:44-157): same recurrences, early exit on ||s||, <t,t>
+    fallback and 1e-30 breakdown guards.  x is updated in place; returns
+    (success, iterations, relative residual)."""
+    tiny = 1e-30                                   # :58
+    t = apply(x)                                   # :61-64
+    r = b - t
+    r_hat = r.copy()                               # :66
+    norm_rhs = np.sqrt(b @ b)                      # :68
+    if norm_rhs == 0.0:
+        x[:] = 0.0
+        return True, 0, 0.0
+    rho_prev = alpha = omega = 1.0                 # :75-77
+    p = np.zeros_like(b)
+    v = np.zeros_like(b)
+    rel_res = np.sqrt(r @ r) / norm_rhs            # :82-83
+    if rel_res < tol:
+        return True, 0, rel_res
+    it = 0
+    while it < max_iter:                           # :90
+        rho = r_hat @ r
+        if abs(rho) < tiny:
+            break
+        if it == 0:
+            p = r.copy()
+        else:
+            beta = (rho / rho_prev) * (alpha / omega)   # :101-104
+            p = r + beta * (p - omega * v)
+        v = apply(p)                               # :107
+        rv = r_hat @ v
+        if abs(rv) < tiny:
+            break
+        alpha = rho / rv                           # :111
+        s = r - alpha * v                          # :113-114
+        rel_s = np.sqrt(s @ s) / norm_rhs
+        if rel_s < tol:                            # :117-124
+            x += alpha * p
+            return True, it + 1, rel_s
+        t = apply(s)                               # :126
+        ts, tt = t @ s, t @ t
+        if abs(tt) < tiny:                         # :130-138
+            x += alpha * p
+            r = s
+            rel_fb = np.sqrt(r @ r) / norm_rhs
+            return rel_fb < tol, it + 1, rel_fb
+        omega = ts / tt                            # :140
+        if abs(omega) < tiny:
+            break
+        x += alpha * p + omega * s                 # :143-144
+        r = s - omega * t                          # :146-147
+        rel_res = np.sqrt(r @ r) / norm_rhs
+        if rel_res < tol:
+            return True, it + 1, rel_res
+        rho_prev = rho
+        it += 1
+    return False, it, rel_res

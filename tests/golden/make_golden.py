@@ -37,6 +37,12 @@ def main():
             digest = hashlib.sha256(x.tobytes()).hexdigest()
             np.savez_compressed(HERE / f"{c.name}.npz", y=y, x_sha256=np.array(digest), n=np.array(x.size))
             print(f"{c.name}: {x.size} dofs, |y|={np.linalg.norm(y):.6e}")
+        # GenDiL's BiCGSTAB_no_alloc on M + 0.05 A (3D, n=4, GLL p=1, 2 points,
+        # C2 field, zero inflow), b = FillRandom(7), x0 = 0, tol 1e-10
+        b, x, it, ok, rel = oracle.ref_bicgstab(3, 1, 2, "L", 4, "c2", "zero", 7, Path(tmp) / "bicgstab.bin")
+        np.savez_compressed(HERE / "bicgstab3d_c2_p1q2_bc0.npz", x=x, iterations=np.array(it), success=np.array(ok),
+                            rel_residual=np.array(rel), b_sha256=np.array(hashlib.sha256(b.tobytes()).hexdigest()))
+        print(f"bicgstab3d_c2_p1q2_bc0: {b.size} dofs, {it} iterations, success {ok}, rel {rel:.3e}")
 
 
 if __name__ == "__main__":
