@@ -257,6 +257,47 @@ def padapt_6d(steps, warmup, n=8):
     }
 
 
+def product_6d(steps, warmup, n=5):
+    """GenDiL's own 6D benchmark operator (benchmarks/benchmark-advection-6d.cpp):
+    3D x 2D x 1D product mesh, n elements per axis (h = 1, 1.2345, 1), Legendre
+    p=1, 3 Gauss points per axis, beta = 1, no boundary data, FP64; beside it
+    the reference's own CPU apply of the same operator where it is built."""
+    import torch
+
+    import gendil_amd as G
+    from gendil_amd.fields import SeparableField, VectorField
+
+    mesh = G.MakeCartesianProductMesh(G.Cartesian3DMesh(1.0, n, n, n), G.Cartesian2DMesh(1.2345, n, n),
+                                      G.Cartesian1DMesh(1.0, n))
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLegendreFiniteElement(1))
+    op = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(3), VectorField([SeparableField.constant(1.0)] * 6))
+    N = op.num_dofs
+    x0 = torch.as_tensor(G.fill_random(N, 12345), device="cuda")
+    xs = [x0, x0 * 0.5, x0 * -0.25]
+    ys = [torch.empty_like(x0) for _ in range(3)]
+    ms = time_applies(op, xs, ys, steps, warmup)
+    out = {
+        "workload": f"GenDiL 6D benchmark operator: product mesh {n}^6 (3D x 2D x 1D), Legendre p=1, q=3, beta=1, FP64",
+        "dofs": N,
+        "value": N / (ms * 1e-3) / 1e9,
+        "unit": "GDoF/s",
+        "ms_per_step": ms,
+        "roofline_frac": 16.0 * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    try:
+        import oracle
+
+        if oracle.ref_available():
+            threads = min(16, os.cpu_count() or 1)
+            r = oracle.ref_bench("adv6dprod", 6, 1, 3, "G", n, False, "ones", "none", 12345, 1,
+                                 env={"OMP_NUM_THREADS": str(threads)})
+            out["cpu_reference"] = {"value": r["dofs_per_s"] / 1e9, "unit": "GDoF/s", "cores": threads,
+                                    "kind": "reference"}
+    except Exception as e:  # the GPU figure stands on its own
+        out["cpu_reference"] = {"error": str(e)[:200]}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -334,6 +375,7 @@ def main():
     if not args.no_padapt:
         result["padapt6d"] = padapt_6d(max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()
+        result["product6d"] = product_6d(max(20, args.steps // 2), 5)
     if not args.no_solve:
         result["solve"] = solve_comparison()
     if not args.no_cpu:

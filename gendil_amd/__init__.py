@@ -6,6 +6,9 @@ tests, benchmarks and the autograd wrapper.
 """
 from . import _lib
 from .discretization import (
+    Cartesian1DMesh,
+    Cartesian2DMesh,
+    Cartesian3DMesh,
     CartesianMesh,
     FiniteElement,
     FiniteElementSpace,
@@ -13,6 +16,7 @@ from .discretization import (
     MakeFiniteElementSpace,
     MakeIntegrationRule,
     MakeLegendreFiniteElement,
+    MakeCartesianProductMesh,
     MakeLobattoFiniteElement,
     MakePAdaptiveSpace,
     axis_points,
@@ -22,6 +26,9 @@ from .fields import AffineField, PointwiseField, SeparableField, VectorField, ce
 from .random import fill_random
 
 __all__ = [
+    "Cartesian1DMesh",
+    "Cartesian2DMesh",
+    "Cartesian3DMesh",
     "CartesianMesh",
     "FiniteElement",
     "FiniteElementSpace",
@@ -29,6 +36,7 @@ __all__ = [
     "MakeFiniteElementSpace",
     "MakeIntegrationRule",
     "MakeLegendreFiniteElement",
+    "MakeCartesianProductMesh",
     "MakeLobattoFiniteElement",
     "MakePAdaptiveSpace",
     "axis_points",

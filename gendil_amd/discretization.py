@@ -2,6 +2,8 @@
 
 These mirror GenDiL's factories so call sites read the same:
   CartesianMesh<D>(sizes, h, origin[, periodic])   cartesianndmesh.hpp:16-80
+  Cartesian{1,2,3}DMesh(h, n...)                   cartesian3dmesh.hpp:32-47 (and 1D/2D)
+  MakeCartesianProductMesh(meshes...)              productmesh.hpp:154-225
   MakeLobattoFiniteElement / MakeLegendreFiniteElement   finiteelement.hpp:103-109
   MakeFiniteElementSpace(mesh, fe)                 finiteelementspace.hpp:69,84
   MakeIntegrationRule(IntegrationRuleNumPoints)    integrationrule.hpp:80
@@ -52,6 +54,44 @@ class CartesianMesh:
     def unit(dim: int, n: int, periodic=False) -> "CartesianMesh":
         """[0,1]^dim with n elements per axis."""
         return CartesianMesh([n] * dim, [1.0 / n] * dim, [0.0] * dim, periodic)
+
+
+def Cartesian1DMesh(h: float, n: int, origin: float = 0.0, periodic: bool = False) -> CartesianMesh:
+    """n cells of size h from origin (Cartesian1DMesh)."""
+    return CartesianMesh([n], [h], [origin], periodic)
+
+
+def Cartesian2DMesh(h: float, nx: int, ny: int, origin=(0.0, 0.0), periodic=False) -> CartesianMesh:
+    """nx x ny square cells of size h (Cartesian2DMesh)."""
+    return CartesianMesh([nx, ny], [h, h], origin, periodic)
+
+
+def Cartesian3DMesh(h: float, nx: int, ny: int, nz: int, origin=(0.0, 0.0, 0.0), periodic=False) -> CartesianMesh:
+    """nx x ny x nz cubic cells of size h (Cartesian3DMesh(h, nx, ny, nz),
+    cartesian3dmesh.hpp:32-36)."""
+    return CartesianMesh([nx, ny, nz], [h, h, h], origin, periodic)
+
+
+def MakeCartesianProductMesh(*meshes: CartesianMesh) -> CartesianMesh:
+    """Tensor product of Cartesian meshes (MakeCartesianProductMesh,
+    productmesh.hpp:154-225) as the Cartesian mesh over the concatenated axes.
+
+    The two index the same: the product numbers cell (head, tail) as
+    head + cells(head) * tail (productmesh.hpp:91-94), which unrolled over
+    the factors is the x-fastest lexicographic order of the concatenated
+    axes; a face normal to axis d of the product is the sub-mesh's face
+    normal to that axis, with the sub-mesh's neighbour and boundary
+    (productmesh.hpp:104-134); cells are products of the factors' cells,
+    so coordinates, sizes and periodicity concatenate."""
+    if not meshes:
+        raise ValueError("MakeCartesianProductMesh needs at least one mesh")
+    sizes, h, origin, periodic = [], [], [], []
+    for m in meshes:
+        sizes += list(m.sizes)
+        h += list(m.h)
+        origin += list(m.origin)
+        periodic += list(m.periodic)
+    return CartesianMesh(sizes, h, origin, periodic)
 
 
 @dataclass(frozen=True)

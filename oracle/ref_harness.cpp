@@ -264,6 +264,29 @@ int run( const Args & a )
    return 2;
 }
 
+// The operator of GenDiL's 6D benchmark (benchmarks/benchmark-advection-6d.cpp:
+// 14-98): a product of a 3D (h = 1), a 2D (h = 1.2345) and a 1D (h = 1)
+// Cartesian mesh with n elements per axis, Legendre p = 1 elements, 3 Gauss
+// points per axis, beta = 1 in every direction, no boundary data.
+int run_product( const Args & a )
+{
+   Cartesian3DMesh mesh_1( 1.0, a.n, a.n, a.n );
+   Cartesian2DMesh mesh_2( 1.2345, a.n, a.n );
+   Cartesian1DMesh mesh_3( 1.0, a.n );
+   auto mesh = MakeCartesianProductMesh( mesh_1, mesh_2, mesh_3 );
+   FiniteElementOrders< 1, 1, 1, 1, 1, 1 > orders;
+   auto fe = MakeLegendreFiniteElement( orders );
+   auto fes = MakeFiniteElementSpace( mesh, fe );
+   IntegrationRuleNumPoints< 3, 3, 3, 3, 3, 3 > nq;
+   auto ir = MakeIntegrationRule( nq );
+   auto adv = []( const std::array< Real, 6 > &, Real ( &v )[6] )
+   {
+      for ( int d = 0; d < 6; ++d ) v[d] = 1.0;
+   };
+   auto op = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv );
+   return finish( a, op, fes.GetNumberOfFiniteElementDofs(), fes.GetNumberOfFiniteElements() );
+}
+
 using Runner = int ( * )( const Args & );
 struct CaseEntry { int D, P, Q; char basis; Runner fn; };
 
@@ -303,6 +326,7 @@ int main( int argc, char ** argv )
    a.seed = (unsigned)std::strtoul( argv[11], nullptr, 10 );
    if ( a.mode == "fixture" || a.mode == "bicgstab" ) a.out = argv[12];
    else a.pairs = std::atoi( argv[12] );
+   if ( a.op == "adv6dprod" ) return run_product( a );
    for ( const CaseEntry & e : kEntries )
    {
       if ( e.D == a.D && e.P == a.P && e.Q == a.Q && e.basis == a.basis[0] ) return e.fn( a );

@@ -33,7 +33,14 @@ class Case:
     seed: int = 12345
 
     def space(self):
-        mesh = G.CartesianMesh.unit(self.D, self.n, self.periodic)
+        if self.op == "adv6dprod":
+            # the mesh of GenDiL's 6D benchmark (benchmark-advection-6d.cpp:14-38)
+            mesh = G.MakeCartesianProductMesh(
+                G.Cartesian3DMesh(1.0, self.n, self.n, self.n), G.Cartesian2DMesh(1.2345, self.n, self.n),
+                G.Cartesian1DMesh(1.0, self.n),
+            )
+        else:
+            mesh = G.CartesianMesh.unit(self.D, self.n, self.periodic)
         fe = G.MakeLobattoFiniteElement(self.P) if self.basis == "L" else G.MakeLegendreFiniteElement(self.P)
         return G.MakeFiniteElementSpace(mesh, fe), G.MakeIntegrationRule(self.Q)
 
@@ -65,6 +72,8 @@ def beta_field(kind: str, D: int):
     if kind == "phase":
         assert D == 6
         return W.phase_space_6d_separable()
+    if kind == "ones":
+        return VectorField([SeparableField.constant(1.0) for _ in range(D)])
     raise ValueError(kind)
 
 
@@ -112,6 +121,9 @@ CASES = [
     Case("adv4d_trig_p2q3_bcpoly", "adv", 4, 2, 3, "L", 3, False, "trig", "poly"),
     Case("adv6d_phase_p1q2_per", "adv", 6, 1, 2, "L", 2, True, "phase", "none"),
     Case("adv6d_phase_p2q3_bc0", "adv", 6, 2, 3, "L", 2, False, "phase", "zero"),
+    # GenDiL's 6D benchmark operator: 3D x 2D x 1D product mesh (h 1, 1.2345, 1),
+    # Legendre p=1, 3 points, beta = 1, no boundary data
+    Case("adv6dprod_ones_p1q3_gl", "adv6dprod", 6, 1, 3, "G", 2, False, "ones", "none"),
 ]
 
 BY_NAME = {c.name: c for c in CASES}
