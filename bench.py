@@ -298,6 +298,61 @@ def product_6d(steps, warmup, n=5):
     return out
 
 
+def adjoint_48(steps, warmup, n=48, p=4, q=5):
+    """The adjoint workload: 3D 48^3 periodic, GLL p=4, 5 Gauss points, FP64,
+    beta per quadrature point (benchmark field + 0.1 N(0,1), seed 7): the
+    forward apply, A^T r and the beta contraction, each timed alone.
+    Algorithmic bytes: the vectors each kernel reads and writes once plus the
+    per-point field (3 q^3 cell values per element, q^2 per face)."""
+    import torch
+
+    import gendil_amd as G
+    from gendil_amd import workloads as W
+
+    mesh = G.CartesianMesh.unit(3, n, True)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(p))
+    ir = G.MakeIntegrationRule(q)
+    cell, faces = W.noisy_pointwise_beta(mesh, q, 7)
+    bc = torch.as_tensor(cell, device="cuda")
+    bf = [torch.as_tensor(f, device="cuda") for f in faces]
+    del cell, faces
+    op = G.MakeAdvectionOperator(fes, ir, G.PointwiseField(bc, bf))
+    plan = op.plan
+    N = op.num_dofs
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    u = torch.rand(N, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    r = torch.rand(N, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    y = torch.empty_like(u)
+    dcell = torch.empty_like(bc)
+    dfaces = [torch.empty_like(f) for f in bf]
+    beta_bytes = 8.0 * (bc.numel() + sum(f.numel() for f in bf))
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    res = {"workload": f"adjoint: 3D {n}^3 periodic, GLL p={p}, {q} GL points, per-point beta, FP64", "dofs": N}
+    for name, fn, nbytes in (
+        ("forward", lambda: plan.apply(0.0, 1.0, u, y), 16.0 * N + beta_bytes),
+        ("transpose", lambda: plan.apply_transpose(0.0, 1.0, r, y), 16.0 * N + beta_bytes),
+        ("grad_beta", lambda: plan.grad_beta(1.0, u, r, dcell, dfaces), 16.0 * N + beta_bytes),
+    ):
+        ms = timed(fn)
+        res[name] = {"ms": ms, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+                     "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    del bc, bf, dcell, dfaces, u, r, y
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -376,6 +431,9 @@ def main():
         result["padapt6d"] = padapt_6d(max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()
         result["product6d"] = product_6d(max(20, args.steps // 2), 5)
+        torch.cuda.empty_cache()
+        result["adjoint48"] = adjoint_48(max(10, args.steps // 10), 2)
+        torch.cuda.empty_cache()
     if not args.no_solve:
         result["solve"] = solve_comparison()
     if not args.no_cpu:
