@@ -86,7 +86,13 @@ struct Cfg {
   static constexpr int SCR = cmax(CELL_SCR, FACE_SCR);
   // per-element LDS: U (input), Y (output or second input), scratch
   static constexpr int PER_ELEM = 2 * ND + SCR;
-  static constexpr int LDS_BUDGET = 64 * 1024;
+  // elements per block: as many as fit in 32 KB of LDS (at most 16).  Smaller
+  // blocks leave more of them resident per CU to hide the sweeps' barriers and
+  // the neighbour-trace loads; measured on MI355X against 16, 24, 48, 64 and
+  // 96 KB (and 128 / 512 threads): 48^3 p=4 q=5 per-point advection 1.07 ->
+  // 0.63 ms, its beta contraction 1.15 -> 0.84 ms, the 6D p=1 q=3 benchmark
+  // operator 1.19 -> 0.89 ms, all relative to 64 KB.
+  static constexpr int LDS_BUDGET = 32 * 1024;
   static constexpr int EPB_RAW = LDS_BUDGET / (int)(PER_ELEM * sizeof(T));
   static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
   static constexpr int NT = 256;
