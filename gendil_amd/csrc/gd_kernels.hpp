@@ -91,11 +91,12 @@ struct Cfg {
   // the neighbour-trace loads; measured on MI355X against 16, 24, 48, 64 and
   // 96 KB (and 128 / 512 threads): 48^3 p=4 q=5 per-point advection 1.07 ->
   // 0.63 ms, its beta contraction 1.15 -> 0.84 ms, the 6D p=1 q=3 benchmark
-  // operator 1.19 -> 0.89 ms, all relative to 64 KB.
+  // operator 1.19 -> 0.89 ms, all relative to 64 KB.  6D blocks take 128
+  // threads (the 6D operator 0.89 -> 0.72 ms; the 3D ones lose with 128).
   static constexpr int LDS_BUDGET = 32 * 1024;
   static constexpr int EPB_RAW = LDS_BUDGET / (int)(PER_ELEM * sizeof(T));
   static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
-  static constexpr int NT = 256;
+  static constexpr int NT = D >= 6 ? 128 : 256;
 };
 
 // ---- pencil sweep --------------------------------------------------------
