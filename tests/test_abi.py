@@ -1,5 +1,6 @@
 """CPU checks of the C-ABI boundary: the library loads, exports every entry
 point include/gendil_amd.h declares, and its host-only helpers behave."""
+import pytest
 import ctypes as C
 import re
 from pathlib import Path
@@ -66,3 +67,30 @@ def test_separable_tabulation_offsets():
     assert s.rank == 2 and s.scale[0] == 2.0
     assert s.offset[0][1] == 0 and s.offset[0][0] == -1 and s.offset[1][1] == -1
     assert data.size == 4 * 5
+
+
+def test_product_mesh_is_the_concatenated_cartesian_mesh():
+    # MakeCartesianProductMesh(3D, 2D, 1D) numbers and sizes cells like the
+    # Cartesian mesh over the concatenated axes (productmesh.hpp:91-94)
+    import gendil_amd as G
+
+    m = G.MakeCartesianProductMesh(G.Cartesian3DMesh(1.0, 2, 3, 4), G.Cartesian2DMesh(1.2345, 5, 6),
+                                   G.Cartesian1DMesh(0.5, 7, origin=-1.0, periodic=True))
+    assert m.sizes == (2, 3, 4, 5, 6, 7)
+    assert m.h == (1.0, 1.0, 1.0, 1.2345, 1.2345, 0.5)
+    assert m.origin == (0.0, 0.0, 0.0, 0.0, 0.0, -1.0)
+    assert m.periodic == (False,) * 5 + (True,)
+    assert m.num_cells == 2 * 3 * 4 * 5 * 6 * 7
+    # cell (head, tail) = head + cells(head) * tail, head x-fastest
+    head, tail = (1, 2, 3), (4, 5, 6)
+    h_idx = head[0] + 2 * (head[1] + 3 * head[2])
+    t_idx = tail[0] + 5 * (tail[1] + 6 * tail[2])
+    e = h_idx + 24 * t_idx
+    sub = []
+    r = e
+    for n in m.sizes:
+        sub.append(r % n)
+        r //= n
+    assert tuple(sub) == head + tail
+    with pytest.raises(ValueError):
+        G.MakeCartesianProductMesh()
