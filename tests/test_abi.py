@@ -1,11 +1,11 @@
 """CPU checks of the C-ABI boundary: the library loads, exports every entry
 point include/gendil_amd.h declares, and its host-only helpers behave."""
-import pytest
 import ctypes as C
 import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 import gendil_amd as G
 from gendil_amd import _lib as L
