@@ -13,6 +13,7 @@ from .discretization import (
     FiniteElement,
     FiniteElementSpace,
     IntegrationRule,
+    L2Restriction,
     MakeFiniteElementSpace,
     MakeIntegrationRule,
     MakeLegendreFiniteElement,
@@ -33,6 +34,7 @@ __all__ = [
     "FiniteElement",
     "FiniteElementSpace",
     "IntegrationRule",
+    "L2Restriction",
     "MakeFiniteElementSpace",
     "MakeIntegrationRule",
     "MakeLegendreFiniteElement",
@@ -62,6 +64,12 @@ def __getattr__(name):
         "MakeMassFiniteElementOperator",
         "MakeAdvectionOperator",
         "MakeMassAdvectionOperator",
+        "AdvectionFaceOperator",
+        "AdvectionNonconformingFaceOperator",
+        "CartesianInteriorFaceConnectivity",
+        "CartesianIntermeshFaceConnectivity",
+        "MakeCartesianInteriorFaceConnectivity",
+        "MakeAdvectionFaceOperator",
     ):
         from . import operators
 

@@ -1,0 +1,358 @@
+// Face-centric upwind advection: the flux of every face computed once and
+// added to the minus element, subtracted from the plus element.
+//
+// Reference: AdvectionLocalFaceOperator / AdvectionNonconformingLocalFaceOperator
+// (advectionoperator.hpp:478-548, 596-676), driven by AdvectionFaceOperator /
+// AdvectionNonconformingFaceOperator (:1105-1180, :1210-1396) over
+// CartesianInteriorLocalFaceConnectivity (cartesianglobalfaceconnectivity.hpp:15-95)
+// or CartesianIntermeshFaceConnectivity (cartesianintermeshfaceconnectivity.hpp:36-180).
+//
+// Per face: both sides' traces (the element contracted along the normal with
+// the side's basis at its face) are interpolated to the q^(D-1) Gauss points
+// of the shared rule, the upwind flux is formed with the minus cell's geometry
+// and field, and B^T of the flux is spread back along the normal on each side.
+// GenDiL accumulates with atomics on the device; here the faces of one launch
+// touch disjoint elements (interior faces of one axis are coloured by the
+// parity of the minus cell's index along that axis), so every update is a
+// plain read-add-write and the result does not depend on scheduling.
+//
+// Layout: a block takes FPB faces; per face four LDS buffers of S doubles
+// (S = max(n_minus, n_plus, q)^(D-1)): ping-pong for the sweeps and the two
+// interpolated traces (the minus one then holds the flux).
+
+#include "gendil_amd.h"
+#include "gd_internal.hpp"
+
+#include <algorithm>
+#include <string>
+
+namespace gd {
+namespace face {
+
+constexpr int kThreads = 256;
+constexpr int kMaxS = 1024;             // largest face tensor, entries
+constexpr int kLdsBytes = 48 * 1024;    // dynamic LDS per block
+
+struct Side {
+  const double* x;
+  double* y;
+  const double* B;   // [q][n] basis at the rule points
+  const double* bf;  // [n] basis at this side's face (xi = 0 or 1)
+  int64_t M[kMaxDim];
+  int n;
+  int64_t base, stride;  // cell index along the axis = base + stride * face row
+};
+
+struct Args {
+  int D, axis, q, sign, fpb, S;
+  int64_t G[kMaxDim];  // face grid, first index fastest
+  int64_t nfaces;
+  Side s[2];           // 0 minus, 1 plus
+  Params<double> geo;  // minus plan: origin, h, detJ, field, rule tables
+  int xi_minus;        // 0: the minus face is at xi = 0, 1: at xi = 1
+};
+
+__device__ __forceinline__ int ipow(int b, int e)
+{
+  int r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+// element index of side `sd` for face f; the tangential cell indices in t[]
+__device__ __forceinline__ int64_t side_elem(const Args& a, const Side& sd, int64_t f, int* sub)
+{
+  int64_t r = f, lin = 0, str = 1;
+  for (int d = 0; d < a.D; ++d) {
+    const int64_t c = r % a.G[d];
+    r /= a.G[d];
+    const int64_t s = d == a.axis ? sd.base + sd.stride * c : c;
+    if (sub) sub[d] = (int)s;
+    lin += s * str;
+    str *= sd.M[d];
+  }
+  return lin;
+}
+
+// element-local dof index of tangential multi-index t (over the D-1
+// tangential axes, first fastest, extent n) with node i along the axis
+__device__ __forceinline__ int dof_index(int D, int axis, int n, int t, int i)
+{
+  int idx = 0, str = 1, r = t;
+  for (int d = 0; d < D; ++d) {
+    int c;
+    if (d == axis) {
+      c = i;
+    } else {
+      c = r % n;
+      r /= n;
+    }
+    idx += c * str;
+    str *= n;
+  }
+  return idx;
+}
+
+// one 1D sweep over tangential axis k of every face of the block:
+// in [outer][a][inner] -> out [outer][b][inner], out = M in, M(j, i) =
+// TRANS ? B[i][j] : B[j][i] with B row-major [q][n]
+template <bool TRANS>
+__device__ void sweep(const double* B, int n, int a, int b, int inner, int outer, int nf, int S, const double* in,
+                      double* out)
+{
+  const int per = outer * b * inner;
+  for (int t = threadIdx.x; t < nf * per; t += kThreads) {
+    const int f = t / per;
+    int r = t - f * per;
+    const int o = r / (b * inner);
+    r -= o * b * inner;
+    const int j = r / inner, c = r - j * inner;
+    const double* src = in + f * S + o * a * inner + c;
+    double s = 0.0;
+    for (int i = 0; i < a; ++i) s += (TRANS ? B[i * n + j] : B[j * n + i]) * src[i * inner];
+    out[f * S + o * b * inner + j * inner + c] = s;
+  }
+}
+
+// the side's trace interpolated to the q^(D-1) points, into `dst`
+__device__ void interp_side(const Args& a, const Side& sd, int64_t f0, int nf, double* A, double* Bb, double* dst)
+{
+  const int T = a.D - 1, n = sd.n, q = a.q, S = a.S;
+  const int nt = ipow(n, T), nd = nt * n;
+  double* first = T == 0 ? dst : A;
+  for (int t = threadIdx.x; t < nf * nt; t += kThreads) {
+    const int f = t / nt, tt = t - f * nt;
+    const int64_t e = side_elem(a, sd, f0 + f, nullptr);
+    const double* u = sd.x + e * nd;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double w = sd.bf[i];
+      if (w != 0.0) s += w * u[dof_index(a.D, a.axis, n, tt, i)];
+    }
+    first[f * S + tt] = s;
+  }
+  __syncthreads();
+  const double* src = A;
+  for (int k = 0; k < T; ++k) {
+    double* out = k == T - 1 ? dst : (src == A ? Bb : A);
+    sweep<false>(sd.B, n, n, q, ipow(q, k), ipow(n, T - 1 - k), nf, S, src, out);
+    __syncthreads();
+    src = out;
+  }
+}
+
+// B^T of the flux (in F) spread back along the normal: y += sgn * bf (x) B^T F
+__device__ void spread_side(const Args& a, const Side& sd, int64_t f0, int nf, const double* F, double* A, double* Bb,
+                            double sgn)
+{
+  const int T = a.D - 1, n = sd.n, q = a.q, S = a.S;
+  const int nt = ipow(n, T), nd = nt * n;
+  const double* src = F;
+  for (int k = 0; k < T; ++k) {
+    double* out = (k % 2 == 0) ? A : Bb;
+    sweep<true>(sd.B, n, q, n, ipow(n, k), ipow(q, T - 1 - k), nf, S, src, out);
+    __syncthreads();
+    src = out;
+  }
+  for (int t = threadIdx.x; t < nf * nd; t += kThreads) {
+    const int f = t / nd, r = t - f * nd;
+    const int i = r / nt, tt = r - i * nt;
+    const double w = sd.bf[i];
+    if (w == 0.0) continue;
+    const int64_t e = side_elem(a, sd, f0 + f, nullptr);
+    double* yy = sd.y + e * nd + dof_index(a.D, a.axis, n, tt, i);
+    *yy += sgn * (w * src[f * S + tt]);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double beta_axis(const Params<double>& p, int c, const int* sub, const int* k, int Q, int D)
+{
+  if (p.beta_kind == GD_BETA_AFFINE) {
+    double v = p.affc[c];
+    for (int d = 0; d < D; ++d) {
+      const double xi = k[d] < Q ? p.tab.xq[k[d]] : (k[d] == Q ? 0.0 : 1.0);
+      const double X = (p.origin[d] + p.h[d] * (double)sub[d]) + p.h[d] * xi;
+      v += p.affA[c][d] * X;
+    }
+    return v;
+  }
+  const FieldDev<double>& fd = p.beta[c];
+  double sum = 0.0;
+  for (int r = 0; r < fd.rank; ++r) {
+    double t = fd.scale[r];
+    for (int d = 0; d < D; ++d) {
+      const double* tb = fd.tab[r][d];
+      if (tb) t *= tb[sub[d] * (Q + 2) + k[d]];
+    }
+    sum += t;
+  }
+  return sum;
+}
+
+__global__ void __launch_bounds__(kThreads) k_faces(Args a)
+{
+  extern __shared__ double lds[];
+  const int64_t f0 = (int64_t)blockIdx.x * a.fpb;
+  const int nf = (int)min((int64_t)a.fpb, a.nfaces - f0);
+  const int S = a.S;
+  double* A = lds;
+  double* Bb = lds + a.fpb * S;
+  double* Um = lds + 2 * a.fpb * S;
+  double* Up = lds + 3 * a.fpb * S;
+  interp_side(a, a.s[0], f0, nf, A, Bb, Um);
+  interp_side(a, a.s[1], f0, nf, A, Bb, Up);
+  // upwind flux at the face points of the minus cell (:512-540)
+  const int T = a.D - 1, q = a.q;
+  const int nq = ipow(q, T);
+  const Params<double>& p = a.geo;
+  const double scale = (double)a.sign / (double)p.h[a.axis];
+  for (int t = threadIdx.x; t < nf * nq; t += kThreads) {
+    const int f = t / nq;
+    int r = t - f * nq;
+    int sub[kMaxDim], k[kMaxDim];
+    (void)side_elem(a, a.s[0], f0 + f, sub);
+    double w = 1.0;
+    for (int d = 0; d < a.D; ++d) {
+      if (d == a.axis) {
+        k[d] = q + a.xi_minus;
+      } else {
+        k[d] = r % q;
+        r /= q;
+        w *= p.tab.w[k[d]];
+      }
+    }
+    const double dot = beta_axis(p, a.axis, sub, k, q, a.D) * scale;
+    const int at = f * S + (t - f * nq);
+    const double up = dot > 0.0 ? dot * Um[at] : dot * Up[at];
+    Um[at] = w * p.detJ * up;
+  }
+  __syncthreads();
+  spread_side(a, a.s[0], f0, nf, Um, A, Bb, 1.0);
+  spread_side(a, a.s[1], f0, nf, Um, A, Bb, -1.0);
+}
+
+int side_setup(const gd_plan* pl, int q, int xi, const double* x, double* y, Side& s)
+{
+  const int n = pl->desc.order + 1;
+  const TableSet* ts = find_tables(pl, n, q);
+  if (!ts) return fail(GD_ERR_UNSUPPORTED, "face operator: plan has no tables for the shared rule");
+  s.x = x;
+  s.y = y;
+  s.B = ts->t64.B;
+  s.bf = xi ? ts->t64.b1 : ts->t64.b0;
+  for (int d = 0; d < kMaxDim; ++d) s.M[d] = d < pl->dim ? pl->desc.sizes[d] : 1;
+  s.n = n;
+  return GD_OK;
+}
+
+int check_plan(const gd_plan* pl, const char* what)
+{
+  if (!pl) return fail(GD_ERR_INVALID, std::string(what) + ": null plan");
+  if (pl->padapt) return fail(GD_ERR_UNSUPPORTED, std::string(what) + ": p-adaptive plans are not supported");
+  return GD_OK;
+}
+
+int launch(Args& a, hipStream_t st)
+{
+  if (a.nfaces <= 0) return GD_OK;
+  const int blocks = (int)((a.nfaces + a.fpb - 1) / a.fpb);
+  const size_t lds = (size_t)4 * a.fpb * a.S * sizeof(double);
+  hipLaunchKernelGGL(k_faces, dim3((unsigned)blocks), dim3(kThreads), lds, st, a);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? GD_OK : hip_fail(e, "face kernel launch");
+}
+
+int common_args(const gd_plan* m, const gd_plan* p, int axis, int sign, Args& a)
+{
+  const int D = m->dim;
+  if (p->dim != D) return fail(GD_ERR_INVALID, "face operator: plans differ in dimension");
+  if (!m->beta_set) return fail(GD_ERR_INVALID, "face operator: advection field not set on the minus plan");
+  if (m->beta_kind == GD_BETA_POINTWISE)
+    return fail(GD_ERR_UNSUPPORTED, "face operator: needs a separable or affine advection field");
+  const int q = m->desc.num_quad;
+  if (p->desc.num_quad != q) return fail(GD_ERR_INVALID, "face operator: both plans must use the same rule");
+  for (int d = 0; d < D; ++d)
+    if (d != axis && m->desc.sizes[d] != p->desc.sizes[d])
+      return fail(GD_ERR_INVALID, "face operator: tangential mesh sizes differ");
+  const int nmax = std::max({m->desc.order + 1, p->desc.order + 1, q});
+  long long S = 1;
+  for (int d = 0; d < D - 1; ++d) S *= nmax;
+  if (S > kMaxS) return fail(GD_ERR_UNSUPPORTED, "face operator: face tensor larger than " + std::to_string(kMaxS));
+  a = Args{};
+  a.D = D;
+  a.axis = axis;
+  a.q = q;
+  a.sign = sign;
+  a.S = (int)S;
+  a.fpb = std::max(1, std::min(64, (int)(kLdsBytes / (4 * S * (long long)sizeof(double)))));
+  a.xi_minus = sign > 0 ? 1 : 0;
+  const TableSet* ts = find_tables(m, m->desc.order + 1, q);
+  if (!ts) return fail(GD_ERR_UNSUPPORTED, "face operator: plan has no tables for its rule");
+  a.geo = make_params<double>(m, ts, 0.0, 1.0);
+  return GD_OK;
+}
+
+}  // namespace face
+}  // namespace gd
+
+extern "C" int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, double* y, gd_stream stream)
+{
+  using namespace gd::face;
+  if (int rc = check_plan(plan, "gd_adv_face_apply_f64")) return rc;
+  if (plan->ndofs > 0 && (!x || !y)) return gd::fail(GD_ERR_INVALID, "gd_adv_face_apply_f64: null vector");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int D = plan->dim;
+  for (int axis = 0; axis < D; ++axis) {
+    // interior faces of this axis, minus cell = the upper one (reference
+    // normal -e_axis), in two colours by the parity of its index
+    for (int c = 0; c < 2; ++c) {
+      Args a;
+      if (int rc = common_args(plan, plan, axis, -1, a)) return rc;
+      const int64_t Ma = plan->desc.sizes[axis];
+      int64_t nf = 1;
+      for (int d = 0; d < D; ++d) {
+        a.G[d] = d == axis ? (Ma - c) / 2 : plan->desc.sizes[d];
+        nf *= a.G[d];
+      }
+      for (int d = D; d < gd::kMaxDim; ++d) a.G[d] = 1;
+      a.nfaces = nf;
+      if (int rc = side_setup(plan, a.q, 0, x, y, a.s[0])) return rc;
+      if (int rc = side_setup(plan, a.q, 1, x, y, a.s[1])) return rc;
+      a.s[0].base = 1 + c;
+      a.s[1].base = c;
+      a.s[0].stride = a.s[1].stride = 2;
+      if (int rc = launch(a, st)) return rc;
+    }
+  }
+  return GD_OK;
+}
+
+extern "C" int gd_adv_intermesh_apply_f64(const gd_plan* minus, const gd_plan* plus, int32_t face,
+                                          const double* x_minus, const double* x_plus, double* y_minus,
+                                          double* y_plus, gd_stream stream)
+{
+  using namespace gd::face;
+  if (int rc = check_plan(minus, "gd_adv_intermesh_apply_f64")) return rc;
+  if (int rc = check_plan(plus, "gd_adv_intermesh_apply_f64")) return rc;
+  const int D = minus->dim;
+  if (face < 0 || face >= 2 * D) return gd::fail(GD_ERR_INVALID, "gd_adv_intermesh_apply_f64: face out of range");
+  if (!x_minus || !x_plus || !y_minus || !y_plus)
+    return gd::fail(GD_ERR_INVALID, "gd_adv_intermesh_apply_f64: null vector");
+  const int axis = face % D, sign = face >= D ? 1 : -1;
+  Args a;
+  if (int rc = common_args(minus, plus, axis, sign, a)) return rc;
+  int64_t nf = 1;
+  for (int d = 0; d < gd::kMaxDim; ++d) {
+    a.G[d] = (d < D && d != axis) ? minus->desc.sizes[d] : 1;
+    nf *= a.G[d];
+  }
+  a.nfaces = nf;
+  if (int rc = side_setup(minus, a.q, sign > 0 ? 1 : 0, x_minus, y_minus, a.s[0])) return rc;
+  if (int rc = side_setup(plus, a.q, sign > 0 ? 0 : 1, x_plus, y_plus, a.s[1])) return rc;
+  // a +axis face: minus cells on the last slice, plus cells on the first
+  a.s[0].base = sign > 0 ? minus->desc.sizes[axis] - 1 : 0;
+  a.s[1].base = sign > 0 ? 0 : plus->desc.sizes[axis] - 1;
+  a.s[0].stride = a.s[1].stride = 0;
+  return launch(a, static_cast<hipStream_t>(stream));
+}

@@ -130,6 +130,9 @@ class FiniteElementSpace:
     # p-adaptive extension: per-element orders / rules (int8 arrays)
     elem_order: Optional[np.ndarray] = None
     elem_quad: Optional[np.ndarray] = None
+    # L2Restriction shift: first DoF of this space in a vector shared by
+    # several spaces (restriction.hpp:38)
+    shift: int = 0
 
     @property
     def dim(self) -> int:
@@ -157,8 +160,17 @@ class FiniteElementSpace:
         return np.concatenate([[0], np.cumsum(nd)]).astype(np.int64)
 
 
-def MakeFiniteElementSpace(mesh: CartesianMesh, fe: FiniteElement) -> FiniteElementSpace:
-    return FiniteElementSpace(mesh, fe)
+@dataclass(frozen=True)
+class L2Restriction:
+    """GenDiL's L2Restriction{shift} (restriction.hpp:38): the space's DoFs
+    start at `shift` in a vector shared with other spaces."""
+
+    shift: int = 0
+
+
+def MakeFiniteElementSpace(mesh: CartesianMesh, fe: FiniteElement, restriction: Optional[L2Restriction] = None
+                           ) -> FiniteElementSpace:
+    return FiniteElementSpace(mesh, fe, shift=int(restriction.shift) if restriction is not None else 0)
 
 
 def MakePAdaptiveSpace(mesh: CartesianMesh, elem_order, elem_quad, basis: int = L.BASIS_GLL) -> FiniteElementSpace:

@@ -32,6 +32,20 @@ def _check_vec(t, n, dtype, name):
         raise ValueError(f"{name}: expected contiguous {dtype} with {n} entries, got {t.dtype} {tuple(t.shape)}")
 
 
+def _span(t, fes, name):
+    """The space's part of `t`: all of it, or the num_dofs entries from the
+    space's L2Restriction shift when `t` is a vector shared by several spaces
+    (MakeReadOnlyElementTensorView on a shifted space, elementtensorview.hpp:42-64)."""
+    n = fes.num_dofs
+    if fes.shift == 0 and isinstance(t, torch.Tensor) and t.numel() == n:
+        return t
+    if not isinstance(t, torch.Tensor) or t.dim() != 1:
+        raise TypeError(f"{name} must be a 1-D torch tensor")
+    if fes.shift < 0 or fes.shift + n > t.numel():
+        raise ValueError(f"{name}: {t.numel()} entries cannot hold {n} DoFs from shift {fes.shift}")
+    return t.narrow(0, fes.shift, n)
+
+
 class Plan:
     """Owns one gd_plan: mesh, basis tables and coefficients on the device."""
 
@@ -140,6 +154,8 @@ class Plan:
 
     # ---- applies ----
     def apply(self, mass_coef: float, adv_coef: float, x: torch.Tensor, y: torch.Tensor):
+        out = y
+        x, y = _span(x, self.fes, "x"), _span(y, self.fes, "y")
         if x.dtype == torch.float64:
             _check_vec(x, self.num_dofs, torch.float64, "x")
             _check_vec(y, self.num_dofs, torch.float64, "y")
@@ -152,7 +168,7 @@ class Plan:
                     "gd_apply_f32")
         else:
             raise TypeError("x must be float64 or float32")
-        return y
+        return out
 
     def apply_transpose(self, mass_coef: float, adv_coef: float, r: torch.Tensor, z: torch.Tensor):
         _check_vec(r, self.num_dofs, torch.float64, "r")
@@ -234,3 +250,111 @@ def MakeMassAdvectionOperator(fes, ir, adv, sigma=None, bc=None, mass_coef=1.0, 
     plan.set_beta(adv)
     plan.set_bc(bc)
     return LinearCombinationOperator(plan, mass_coef, adv_coef)
+
+
+# ---- face-centric operators (accumulating) ----------------------------------
+
+class CartesianInteriorFaceConnectivity:
+    """The interior faces of a Cartesian mesh, every axis
+    (MakeCartesianInteriorFaceConnectivity, cartesianglobalfaceconnectivity.hpp:
+    89-108): no periodic wrap, no boundary faces."""
+
+    def __init__(self, sizes):
+        self.sizes = tuple(int(s) for s in sizes)
+
+
+def MakeCartesianInteriorFaceConnectivity(sizes) -> CartesianInteriorFaceConnectivity:
+    return CartesianInteriorFaceConnectivity(sizes)
+
+
+class CartesianIntermeshFaceConnectivity:
+    """Face `face` (GenDiL's local numbering: 0..D-1 the -axis faces,
+    D..2D-1 the +axis faces) of the minus mesh glued to the opposite face of
+    the plus mesh (cartesianintermeshfaceconnectivity.hpp:36-180)."""
+
+    def __init__(self, dim: int, face: int, sizes_minus, sizes_plus):
+        self.dim, self.face = int(dim), int(face)
+        if not 0 <= self.face < 2 * self.dim:
+            raise ValueError("face out of range")
+        self.sizes_minus = tuple(int(s) for s in sizes_minus)
+        self.sizes_plus = tuple(int(s) for s in sizes_plus)
+        axis = self.face % self.dim
+        for d in range(self.dim):
+            if d != axis and self.sizes_minus[d] != self.sizes_plus[d]:
+                raise ValueError("tangential sizes of the two meshes must match")
+
+
+class AdvectionFaceOperator:
+    """y += the upwind fluxes of every interior face, computed once per face
+    (AdvectionFaceOperator, advectionoperator.hpp:1105-1180).  Like GenDiL's,
+    op(x, y) ACCUMULATES into y (read-write view, :1173-1179)."""
+
+    def __init__(self, plan: Plan):
+        self.plan = plan
+
+    @property
+    def num_dofs(self):
+        return self.plan.num_dofs
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor):
+        xs, ys = _span(x, self.plan.fes, "x"), _span(y, self.plan.fes, "y")
+        _check_vec(xs, self.plan.num_dofs, torch.float64, "x")
+        _check_vec(ys, self.plan.num_dofs, torch.float64, "y")
+        if xs.data_ptr() == ys.data_ptr():
+            raise ValueError("x and y must not overlap")
+        L.check(L.lib().gd_adv_face_apply_f64(self.plan.handle, xs.data_ptr(), ys.data_ptr(), _stream()),
+                "gd_adv_face_apply_f64")
+        return y
+
+    Apply = __call__
+
+
+class AdvectionNonconformingFaceOperator:
+    """Fluxes across the interface of two meshes whose spaces may differ in
+    order (AdvectionNonconformingFaceOperator, advectionoperator.hpp:1210-1396):
+    + on the minus space's part of y, - on the plus space's, accumulated.  The
+    two spaces live in one vector through their L2Restriction shifts."""
+
+    def __init__(self, minus: Plan, plus: Plan, face: int):
+        self.minus, self.plus, self.face = minus, plus, int(face)
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor):
+        fm, fp = self.minus.fes, self.plus.fes
+        xm, xp = _span(x, fm, "x"), _span(x, fp, "x")
+        ym, yp = _span(y, fm, "y"), _span(y, fp, "y")
+        for t, n, name in ((xm, fm.num_dofs, "x"), (xp, fp.num_dofs, "x"), (ym, fm.num_dofs, "y"),
+                           (yp, fp.num_dofs, "y")):
+            _check_vec(t, n, torch.float64, name)
+        L.check(
+            L.lib().gd_adv_intermesh_apply_f64(self.minus.handle, self.plus.handle, self.face, xm.data_ptr(),
+                                               xp.data_ptr(), ym.data_ptr(), yp.data_ptr(), _stream()),
+            "gd_adv_intermesh_apply_f64",
+        )
+        return y
+
+    Apply = __call__
+
+
+def MakeAdvectionFaceOperator(*args):
+    """MakeAdvectionFaceOperator(fes, face_mesh, ir, adv) or
+    MakeAdvectionFaceOperator(fes_minus, fes_plus, face_mesh, ir, adv)
+    (advectionoperator.hpp:1546-1607); no boundary data, as in GenDiL."""
+    if len(args) == 4:
+        fes, conn, ir, adv = args
+        if not isinstance(conn, CartesianInteriorFaceConnectivity):
+            raise TypeError("single-space face operators take MakeCartesianInteriorFaceConnectivity(sizes)")
+        if tuple(conn.sizes) != tuple(fes.mesh.sizes):
+            raise ValueError("face connectivity sizes differ from the mesh")
+        plan = Plan(fes, ir)
+        plan.set_beta(adv)
+        return AdvectionFaceOperator(plan)
+    if len(args) == 5:
+        fm, fp, conn, ir, adv = args
+        if not isinstance(conn, CartesianIntermeshFaceConnectivity):
+            raise TypeError("two-space face operators take a CartesianIntermeshFaceConnectivity")
+        if tuple(conn.sizes_minus) != tuple(fm.mesh.sizes) or tuple(conn.sizes_plus) != tuple(fp.mesh.sizes):
+            raise ValueError("face connectivity sizes differ from the meshes")
+        pm, pp = Plan(fm, ir), Plan(fp, ir)
+        pm.set_beta(adv)
+        return AdvectionNonconformingFaceOperator(pm, pp, conn.face)
+    raise TypeError("MakeAdvectionFaceOperator takes 4 or 5 arguments")

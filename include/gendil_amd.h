@@ -140,6 +140,30 @@ int gd_adv_grad_beta_f64(const gd_plan* plan, double adv_coef, const double* u,
                          const double* r, double* dbeta_cell, double* const* dbeta_face,
                          gd_stream stream);
 
+/* ---- face-centric advection (accumulating) ------------------------------ */
+/* y += the upwind face terms of every interior face of the plan's Cartesian
+ * mesh, each face computed once: + on the minus (upper) cell, - on the plus
+ * (lower) cell.  No cell term, no boundary or periodic-wrap faces.  Replaces
+ * AdvectionFaceOperator over CartesianInteriorFaceConnectivity
+ * (advectionoperator.hpp:1105-1180, 478-548; cartesianglobalfaceconnectivity.hpp:
+ * 15-95), built by MakeAdvectionFaceOperator(fes, face_mesh, ir, adv)
+ * (:1546-1565).  Uniform plans, separable or affine beta; x and y must not
+ * overlap. */
+int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, double* y, gd_stream stream);
+/* Face `face` (0..2*dim-1, GenDiL's local face numbering) of the minus plan's
+ * mesh glued to the opposite face of the plus plan's mesh: y_minus += and
+ * y_plus -= the upwind fluxes on the interface, both sides interpolated to
+ * the shared rule (both plans must have the same num_quad; their orders may
+ * differ), geometry and beta from the minus plan.  Replaces
+ * AdvectionNonconformingFaceOperator over CartesianIntermeshFaceConnectivity
+ * <dim, face> (advectionoperator.hpp:578-676, 1210-1396; cartesianintermeshface
+ * connectivity.hpp:36-180), built by MakeAdvectionFaceOperator(fes_minus,
+ * fes_plus, face_mesh, ir, adv) (:1586-1607).  The four vectors may be parts
+ * of one vector (GenDiL's L2Restriction shift). */
+int gd_adv_intermesh_apply_f64(const gd_plan* minus, const gd_plan* plus, int32_t face,
+                               const double* x_minus, const double* x_plus, double* y_minus,
+                               double* y_plus, gd_stream stream);
+
 /* ---- kernel variant selection (autotuner) ------------------------------ */
 /* Times every compiled variant of the apply for this plan on scratch
  * vectors, keeps the fastest for later applies of that precision (4 or 8

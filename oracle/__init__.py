@@ -317,3 +317,139 @@ def bicgstab(apply, b, x, max_iter, tol):
         rho_prev = rho
         it += 1
     return False, it, rel_res
+
+
+# ---------------------------------------------------------------------------
+# Face-centric upwind advection (numpy restatement, small meshes only)
+# ---------------------------------------------------------------------------
+
+def lagrange_matrix(nodes, x):
+    """L[k, i] = l_i(x_k) for the Lagrange polynomials on `nodes`, product
+    form (lagrangeshapefunctions.hpp:37-86)."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    L = np.ones((x.size, nodes.size))
+    for i in range(nodes.size):
+        for j in range(nodes.size):
+            if j != i:
+                L[:, i] *= (x - nodes[j]) / (nodes[i] - nodes[j])
+    return L
+
+
+def interior_face_pairs(sizes, axis):
+    """(minus, plus) element pairs of CartesianInteriorLocalFaceConnectivity
+    <D, axis> (cartesianglobalfaceconnectivity.hpp:64-87): faces enumerated
+    over the grid with one fewer cell along `axis`, first index fastest; the
+    plus cell is the lower one, the minus cell the upper one (reference normal
+    -e_axis of the minus cell)."""
+    sizes = list(sizes)
+    fs = sizes.copy()
+    fs[axis] -= 1
+    pairs = []
+    for f in range(int(np.prod(fs))):
+        idx = list(np.unravel_index(f, fs, order="F"))
+        plus = int(np.ravel_multi_index(idx, sizes, order="F"))
+        idx[axis] += 1
+        minus = int(np.ravel_multi_index(idx, sizes, order="F"))
+        pairs.append((minus, plus))
+    return pairs
+
+
+def intermesh_face_pairs(sizes_m, sizes_p, face):
+    """(minus, plus) pairs of CartesianIntermeshFaceConnectivity<D, face>
+    (cartesianintermeshfaceconnectivity.hpp:150-178): tangential positions
+    first index fastest; a +axis face puts the minus cells on the last slice
+    and the plus cells on the first, a -axis face the other way round."""
+    D = len(sizes_m)
+    axis, sign = face % D, (1 if face >= D else -1)
+    fs = list(sizes_m)
+    fs[axis] = 1
+    pairs = []
+    for f in range(int(np.prod(fs))):
+        t = list(np.unravel_index(f, fs, order="F"))
+        im, ip = t.copy(), t.copy()
+        im[axis] = sizes_m[axis] - 1 if sign > 0 else 0
+        ip[axis] = 0 if sign > 0 else sizes_p[axis] - 1
+        pairs.append((int(np.ravel_multi_index(im, sizes_m, order="F")),
+                      int(np.ravel_multi_index(ip, sizes_p, order="F"))))
+    return pairs
+
+
+def advection_faces(axis, sign, q, minus, plus, pairs, beta, u_m, u_p, y_m, y_p):
+    """Accumulate the upwind face fluxes of GenDiL's AdvectionLocalFaceOperator
+    / AdvectionNonconformingLocalFaceOperator (advectionoperator.hpp:478-548,
+    596-676) for each (minus, plus) element pair into y_m (+) and y_p (-).
+
+    minus / plus: dicts with `sizes`, `h`, `origin` (per axis), `n` (nodes
+    per axis) and `basis` (0 GLL, 1 GL nodes).  The face carries the minus
+    side's reference normal sign * e_axis.  Both sides are interpolated to the
+    same q Gauss points per tangential axis (the shared rule, :1210-1229);
+    the normal direction uses the side's basis at its face (ZeroPoint /
+    OnePoint, faceintegrationrules.hpp:29-56).  Geometry, weight and beta come
+    from the minus cell: X = origin + h (sub + xi), detJ = prod h, physical
+    normal = sign e_axis / h_axis (computephysicalnormal.hpp:27-35), flux =
+    w detJ (dot > 0 ? dot u- : dot u+) (:533-540).  beta maps an array
+    [..., D] of points to [..., D]."""
+    D = len(minus["sizes"])
+    xq, wq = points(1, q)
+    tdims = [d for d in range(D) if d != axis]
+    sides = []
+    for s, xi in ((minus, 1.0 if sign > 0 else 0.0), (plus, 0.0 if sign > 0 else 1.0)):
+        nodes, _ = points(s["basis"], s["n"])
+        sides.append((s["n"], lagrange_matrix(nodes, xq), lagrange_matrix(nodes, [xi])[0], xi))
+    (nm, Bm, bm, xim), (npl, Bp, bp, _) = sides
+    hm = np.asarray(minus["h"], dtype=np.float64)
+    om = np.asarray(minus["origin"], dtype=np.float64)
+    detJ = float(np.prod(hm))
+    # face points (tangential axes in increasing order, first fastest -> numpy axis order)
+    grids = np.meshgrid(*([xq] * (D - 1)), indexing="ij")
+    wts = np.ones([q] * (D - 1))
+    for k in range(D - 1):
+        shape = [1] * (D - 1)
+        shape[k] = q
+        wts = wts * wq.reshape(shape)
+
+    def trace(u, n, B, b):
+        t = np.tensordot(u.reshape([n] * D, order="F"), b, axes=([axis], [0]))
+        for k in range(D - 1):
+            t = np.moveaxis(np.tensordot(B, t, axes=([1], [k])), 0, k)
+        return t
+
+    def spread(F, n, B, b):
+        t = F
+        for k in range(D - 1):
+            t = np.moveaxis(np.tensordot(B.T, t, axes=([1], [k])), 0, k)
+        full = np.multiply.outer(t, b)                 # normal axis last
+        full = np.moveaxis(full, D - 1, axis)
+        return full.reshape(-1, order="F")
+
+    for em, ep in pairs:
+        sub = np.unravel_index(em, minus["sizes"], order="F")
+        X = np.zeros([q] * (D - 1) + [D])
+        for k, d in enumerate(tdims):
+            X[..., d] = om[d] + hm[d] * (sub[d] + grids[k])
+        X[..., axis] = om[axis] + hm[axis] * (sub[axis] + xim)
+        dot = beta(X)[..., axis] * (sign / hm[axis])
+        um_q = trace(u_m[em * nm**D:(em + 1) * nm**D], nm, Bm, bm)
+        up_q = trace(u_p[ep * npl**D:(ep + 1) * npl**D], npl, Bp, bp)
+        F = wts * detJ * np.where(dot > 0.0, dot * um_q, dot * up_q)
+        y_m[em * nm**D:(em + 1) * nm**D] += spread(F, nm, Bm, bm)
+        y_p[ep * npl**D:(ep + 1) * npl**D] -= spread(F, npl, Bp, bp)
+
+
+def face_operator(sizes, h, origin, n, basis, q, beta, u, y):
+    """GenDiL's AdvectionFaceOperator over the Cartesian interior faces of one
+    space (advectionoperator.hpp:1105-1180): every interior face of every
+    axis once, accumulated into y."""
+    side = dict(sizes=list(sizes), h=list(h), origin=list(origin), n=n, basis=basis)
+    for axis in range(len(sizes)):
+        advection_faces(axis, -1, q, side, side, interior_face_pairs(sizes, axis), beta, u, u, y, y)
+
+
+def intermesh_operator(face, q, minus, plus, beta, u_m, u_p, y_m, y_p):
+    """GenDiL's AdvectionNonconformingFaceOperator over
+    CartesianIntermeshFaceConnectivity<D, face> (advectionoperator.hpp:
+    1210-1396), accumulated into y_m and y_p."""
+    D = len(minus["sizes"])
+    pairs = intermesh_face_pairs(minus["sizes"], plus["sizes"], face)
+    advection_faces(face % D, 1 if face >= D else -1, q, minus, plus, pairs, beta, u_m, u_p, y_m, y_p)

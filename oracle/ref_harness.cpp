@@ -14,9 +14,11 @@
 // Usage:
 //   gendil_ref fixture <op> <D> <P> <Q> <basis> <n> <periodic> <coef> <bc> <seed> <out.bin>
 //   gendil_ref bench   <op> <D> <P> <Q> <basis> <n> <periodic> <coef> <bc> <seed> <pairs>
-//   op    : mass | adv
+//   op    : mass | adv | advface (face-centric operator on the interior faces)
+//           | intermesh_<F>_<PP> (3D, GLL: face F of an n^3 minus mesh of order
+//           P glued to an n^3 plus mesh of order PP, q shared; vector [minus|plus])
 //   basis : L (Gauss-Lobatto nodes) | G (Gauss-Legendre nodes)
-//   coef  : mass -> one | poly ;  adv -> c2 | bubble | const | trig | phase
+//   coef  : mass -> one | poly ;  adv -> c2 | bubble | const | trig | phase | yzx
 //   bc    : none (boundary faces skipped) | zero | poly
 // fixture output: int64 ndofs, then x[ndofs] (FillRandom(seed)), then y[ndofs].
 
@@ -70,6 +72,16 @@ void advection_field( const std::string & kind, const std::array< Real, D > & X,
    else if ( kind == "trig" )
    {
       for ( int d = 0; d < D; ++d ) v[d] = 0.5 + std::sin( 2.0 * kPi * X[( d + 1 ) % D] );
+   }
+   else if ( kind == "yzx" )
+   {
+      // tests/integration/gendil/test-advection-p-adaptivity.cpp:62-71
+      if constexpr ( D == 3 )
+      {
+         v[0] = X[1];
+         v[1] = X[2];
+         v[2] = X[0];
+      }
    }
    else if ( kind == "phase" )
    {
@@ -260,9 +272,59 @@ int run( const Args & a )
       auto op = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv, g );
       return finish( a, op, ndofs, nelem );
    }
+   if ( a.op == "advface" )
+   {
+      // AdvectionFaceOperator over CartesianInteriorFaceConnectivity
+      // (advectionoperator.hpp:1105-1180, cartesianglobalfaceconnectivity.hpp:15-95):
+      // every interior face once, y accumulated (+ minus side, - plus side)
+      auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+      std::array< GlobalIndex, D > sizes;
+      for ( int d = 0; d < D; ++d ) sizes[d] = a.n;
+      auto conn = MakeCartesianInteriorFaceConnectivity< D >( sizes );
+      auto op = MakeAdvectionFaceOperator< SerialKernelConfiguration >( fes, conn, ir, adv );
+      return finish( a, op, ndofs, nelem );
+   }
    std::fprintf( stderr, "unknown op %s\n", a.op.c_str() );
    return 2;
 }
+
+using Runner = int ( * )( const Args & );
+
+// GenDiL's nonconforming face operator (advectionoperator.hpp:578-676,
+// 1210-1396) over CartesianIntermeshFaceConnectivity<3, F>: face F of an n^3
+// minus mesh of GLL order PM at the origin glued to an n^3 plus mesh of order
+// PP shifted by one unit along the face normal, Q points shared by both
+// sides, the minus space first in the vector (as
+// tests/integration/gendil/test-advection-p-adaptivity.cpp:13-100).
+template < int PM, int PP, int Q, int F >
+int run_intermesh( const Args & a )
+{
+   constexpr int D = 3;
+   constexpr int axis = F % D;
+   const Real h = 1.0 / a.n;
+   Point< D > om = { 0.0, 0.0, 0.0 };
+   Point< D > opl = om;
+   opl[axis] = F >= D ? 1.0 : -1.0;
+   Cartesian3DMesh mesh_m( h, h, h, a.n, a.n, a.n, om );
+   Cartesian3DMesh mesh_p( h, h, h, a.n, a.n, a.n, opl );
+   auto fe_m = MakeLobattoFiniteElement( FiniteElementOrders< PM, PM, PM >{} );
+   auto fe_p = MakeLobattoFiniteElement( FiniteElementOrders< PP, PP, PP >{} );
+   L2Restriction res_m{ 0 };
+   auto fes_m = MakeFiniteElementSpace( mesh_m, fe_m, res_m );
+   const Integer nm = fes_m.GetNumberOfFiniteElementDofs();
+   L2Restriction res_p{ nm };
+   auto fes_p = MakeFiniteElementSpace( mesh_p, fe_p, res_p );
+   const Integer np = fes_p.GetNumberOfFiniteElementDofs();
+   auto ir = MakeIntegrationRule( IntegrationRuleNumPoints< Q, Q, Q >{} );
+   const std::array< GlobalIndex, D > sizes{ (GlobalIndex)a.n, (GlobalIndex)a.n, (GlobalIndex)a.n };
+   CartesianIntermeshFaceConnectivity< D, F > conn( sizes, sizes );
+   const std::string coef = a.coef;
+   auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+   auto op = MakeAdvectionFaceOperator< SerialKernelConfiguration >( fes_m, fes_p, conn, ir, adv );
+   return finish( a, op, nm + np, fes_m.GetNumberOfFiniteElements() + fes_p.GetNumberOfFiniteElements() );
+}
+
+struct IntermeshEntry { int PM, PP, Q, F; Runner fn; };
 
 // The operator of GenDiL's 6D benchmark (benchmarks/benchmark-advection-6d.cpp:
 // 14-98): a product of a 3D (h = 1), a 2D (h = 1.2345) and a 1D (h = 1)
@@ -287,7 +349,6 @@ int run_product( const Args & a )
    return finish( a, op, fes.GetNumberOfFiniteElementDofs(), fes.GetNumberOfFiniteElements() );
 }
 
-using Runner = int ( * )( const Args & );
 struct CaseEntry { int D, P, Q; char basis; Runner fn; };
 
 #define GD_CASE( D, P, Q ) { D, P, Q, 'L', &run< D, P, Q, true > }
@@ -301,6 +362,12 @@ const CaseEntry kEntries[] = {
    GD_CASE( 1, 3, 4 ), GD_CASE( 2, 2, 3 ), GD_CASE( 2, 3, 4 ),
    GD_CASE( 4, 1, 2 ), GD_CASE( 4, 2, 3 ),
    GD_CASE( 6, 1, 2 ), GD_CASE( 6, 2, 3 ),
+};
+
+const IntermeshEntry kIntermesh[] = {
+   { 1, 3, 5, 3, &run_intermesh< 1, 3, 5, 3 > }, { 3, 1, 5, 0, &run_intermesh< 3, 1, 5, 0 > },
+   { 2, 2, 4, 0, &run_intermesh< 2, 2, 4, 0 > }, { 2, 3, 4, 4, &run_intermesh< 2, 3, 4, 4 > },
+   { 3, 2, 4, 2, &run_intermesh< 3, 2, 4, 2 > },
 };
 
 } // namespace
@@ -327,6 +394,15 @@ int main( int argc, char ** argv )
    if ( a.mode == "fixture" || a.mode == "bicgstab" ) a.out = argv[12];
    else a.pairs = std::atoi( argv[12] );
    if ( a.op == "adv6dprod" ) return run_product( a );
+   if ( a.op.rfind( "intermesh_", 0 ) == 0 )
+   {
+      int F = -1, PP = -1;
+      if ( std::sscanf( a.op.c_str(), "intermesh_%d_%d", &F, &PP ) != 2 ) return 2;
+      for ( const IntermeshEntry & e : kIntermesh )
+         if ( e.PM == a.P && e.PP == PP && e.Q == a.Q && e.F == F ) return e.fn( a );
+      std::fprintf( stderr, "intermesh case F=%d PM=%d PP=%d Q=%d not instantiated\n", F, a.P, PP, a.Q );
+      return 2;
+   }
    for ( const CaseEntry & e : kEntries )
    {
       if ( e.D == a.D && e.P == a.P && e.Q == a.Q && e.basis == a.basis[0] ) return e.fn( a );

@@ -74,6 +74,26 @@ def beta_field(kind: str, D: int):
         return W.phase_space_6d_separable()
     if kind == "ones":
         return VectorField([SeparableField.constant(1.0) for _ in range(D)])
+    if kind == "yzx":  # test-advection-p-adaptivity.cpp:62-71
+        assert D == 3
+        return G.AffineField(np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0]]), np.zeros(3))
+    raise ValueError(kind)
+
+
+def beta_numpy(kind: str, D: int):
+    """The same fields as functions of points [..., D] (for the numpy face oracle)."""
+    if kind == "c2":
+        return lambda X: np.stack([1 + 0.5 * np.sin(TWO_PI * X[..., 1]), 0.5 + 0.5 * np.cos(TWO_PI * X[..., 2]),
+                                   0.25 + 0.25 * np.sin(TWO_PI * X[..., 0])], -1)
+    if kind == "trig":
+        return lambda X: np.stack([0.5 + np.sin(TWO_PI * X[..., (d + 1) % D]) for d in range(D)], -1)
+    if kind == "const":
+        return lambda X: np.stack([np.full(X.shape[:-1], (-0.5) ** d) for d in range(D)], -1)
+    if kind == "phase":
+        return lambda X: np.stack([X[..., 3], X[..., 4], X[..., 5], np.full(X.shape[:-1], 0.1),
+                                   np.zeros(X.shape[:-1]), np.zeros(X.shape[:-1])], -1)
+    if kind == "yzx":
+        return lambda X: np.stack([X[..., 1], X[..., 2], X[..., 0]], -1)
     raise ValueError(kind)
 
 
@@ -127,6 +147,87 @@ CASES = [
 ]
 
 BY_NAME = {c.name: c for c in CASES}
+
+
+@dataclass(frozen=True)
+class FaceCase:
+    """A face-centric operator of ref_harness.cpp: `interior` = advface on the
+    unit n^D mesh (non-periodic); `intermesh` = face F of an n^3 GLL mesh of
+    order P at the origin glued to an n^3 mesh of order PP shifted by one unit
+    along the face normal, q shared, vector [minus | plus]."""
+
+    name: str
+    kind: str  # interior | intermesh
+    D: int
+    P: int
+    Q: int
+    basis: str
+    n: int
+    coef: str
+    PP: int = 0
+    face: int = 0
+    seed: int = 12345
+
+    @property
+    def harness_op(self):
+        return "advface" if self.kind == "interior" else f"intermesh_{self.face}_{self.PP}"
+
+    def sides(self):
+        """(minus, plus) oracle side descriptions."""
+        h = 1.0 / self.n
+        b = 0 if self.basis == "L" else 1
+        m = dict(sizes=[self.n] * self.D, h=[h] * self.D, origin=[0.0] * self.D, n=self.P + 1, basis=b)
+        if self.kind == "interior":
+            return m, m
+        po = [0.0] * self.D
+        po[self.face % self.D] = 1.0 if self.face >= self.D else -1.0
+        return m, dict(sizes=[self.n] * self.D, h=[h] * self.D, origin=po, n=self.PP + 1, basis=b)
+
+    def oracle(self, x):
+        import oracle
+
+        beta = beta_numpy(self.coef, self.D)
+        m, p = self.sides()
+        y = np.zeros_like(x)
+        if self.kind == "interior":
+            oracle.face_operator(m["sizes"], m["h"], m["origin"], m["n"], m["basis"], self.Q, beta, x, y)
+        else:
+            nm = self.n ** self.D * m["n"] ** self.D
+            oracle.intermesh_operator(self.face, self.Q, m, p, beta, x[:nm], x[nm:], y[:nm], y[nm:])
+        return y
+
+    def gpu_operator(self):
+        """The operator built through gendil_amd's GenDiL-shaped factories."""
+        fe = G.MakeLobattoFiniteElement if self.basis == "L" else G.MakeLegendreFiniteElement
+        ir = G.MakeIntegrationRule(self.Q)
+        beta = beta_field(self.coef, self.D)
+        if self.kind == "interior":
+            fes = G.MakeFiniteElementSpace(G.CartesianMesh.unit(self.D, self.n, False), fe(self.P))
+            return G.MakeAdvectionFaceOperator(fes, G.MakeCartesianInteriorFaceConnectivity(fes.mesh.sizes), ir, beta)
+        m, p = self.sides()
+        mesh_m = G.Cartesian3DMesh(1.0 / self.n, self.n, self.n, self.n, origin=tuple(m["origin"]))
+        mesh_p = G.Cartesian3DMesh(1.0 / self.n, self.n, self.n, self.n, origin=tuple(p["origin"]))
+        fm = G.MakeFiniteElementSpace(mesh_m, fe(self.P))
+        fp = G.MakeFiniteElementSpace(mesh_p, fe(self.PP), G.L2Restriction(fm.num_dofs))
+        conn = G.CartesianIntermeshFaceConnectivity(3, self.face, mesh_m.sizes, mesh_p.sizes)
+        return G.MakeAdvectionFaceOperator(fm, fp, conn, ir, beta)
+
+
+FACE_CASES = [
+    FaceCase("face3d_c2_p3q4", "interior", 3, 3, 4, "L", 4, "c2"),
+    FaceCase("face3d_trig_gl_p2q3", "interior", 3, 2, 3, "G", 3, "trig"),
+    FaceCase("face2d_trig_p2q3", "interior", 2, 2, 3, "L", 4, "trig"),
+    FaceCase("face6d_phase_p1q2", "interior", 6, 1, 2, "L", 2, "phase"),
+    # test-advection-p-adaptivity.cpp: +x face, p = 1 | p = 3, q = 5, beta = (y, z, x)
+    FaceCase("imesh3d_yzx_f3_p1p3q5", "intermesh", 3, 1, 5, "L", 3, "yzx", PP=3, face=3),
+    FaceCase("imesh3d_yzx_f0_p3p1q5", "intermesh", 3, 3, 5, "L", 3, "yzx", PP=1, face=0),
+    # test-face-advection-split-mesh.cpp: -x face, p = 2 both sides, q = 4
+    FaceCase("imesh3d_const_f0_p2p2q4", "intermesh", 3, 2, 4, "L", 3, "const", PP=2, face=0),
+    FaceCase("imesh3d_c2_f4_p2p3q4", "intermesh", 3, 2, 4, "L", 3, "c2", PP=3, face=4),
+    FaceCase("imesh3d_c2_f2_p3p2q4", "intermesh", 3, 3, 4, "L", 3, "c2", PP=2, face=2),
+]
+
+FACE_BY_NAME = {c.name: c for c in FACE_CASES}
 
 
 def oracle_problem(case: Case, **kw):

@@ -24,7 +24,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 import oracle  # noqa: E402
-from cases import CASES  # noqa: E402
+from cases import CASES, FACE_CASES  # noqa: E402
 
 
 def main():
@@ -34,6 +34,14 @@ def main():
         for c in CASES:
             path = Path(tmp) / f"{c.name}.bin"
             x, y = oracle.ref_fixture(c.op, c.D, c.P, c.Q, c.basis, c.n, c.periodic, c.coef, c.bc, c.seed, path)
+            digest = hashlib.sha256(x.tobytes()).hexdigest()
+            np.savez_compressed(HERE / f"{c.name}.npz", y=y, x_sha256=np.array(digest), n=np.array(x.size))
+            print(f"{c.name}: {x.size} dofs, |y|={np.linalg.norm(y):.6e}")
+        # face-centric operators (GenDiL's AdvectionFaceOperator and its
+        # nonconforming variant), y accumulated from zero
+        for c in FACE_CASES:
+            path = Path(tmp) / f"{c.name}.bin"
+            x, y = oracle.ref_fixture(c.harness_op, c.D, c.P, c.Q, c.basis, c.n, 0, c.coef, "none", c.seed, path)
             digest = hashlib.sha256(x.tobytes()).hexdigest()
             np.savez_compressed(HERE / f"{c.name}.npz", y=y, x_sha256=np.array(digest), n=np.array(x.size))
             print(f"{c.name}: {x.size} dofs, |y|={np.linalg.norm(y):.6e}")
