@@ -31,7 +31,8 @@ namespace face {
 
 constexpr int kThreads = 256;
 constexpr int kMaxS = 1024;             // largest face tensor, entries
-constexpr int kLdsBytes = 48 * 1024;    // dynamic LDS per block
+constexpr int kLdsBytes = 40 * 1024;    // dynamic LDS per block
+constexpr int kMaxFpb = 64;             // faces per block
 
 struct Side {
   const double* x;
@@ -97,7 +98,8 @@ __device__ __forceinline__ int dof_index(int D, int axis, int n, int t, int i)
 // in [outer][a][inner] -> out [outer][b][inner], out = M in, M(j, i) =
 // TRANS ? B[i][j] : B[j][i] with B row-major [q][n]
 template <bool TRANS>
-__device__ void sweep(const double* B, int n, int a, int b, int inner, int outer, int nf, int S, const double* in,
+__device__ __forceinline__ void sweep(const double* B, int n, int a, int b, int inner, int outer, int nf, int S,
+                                      const double* in,
                       double* out)
 {
   const int per = outer * b * inner;
@@ -114,20 +116,29 @@ __device__ void sweep(const double* B, int n, int a, int b, int inner, int outer
   }
 }
 
+// Per-block index tables in LDS: each face's element on both sides, and per
+// side the element-local offset of every tangential node (node 0 along the
+// axis; node i adds i * n^axis)
+struct Idx {
+  int64_t elem[2][kMaxFpb];
+  int off[2][kMaxS];
+};
+
 // the side's trace interpolated to the q^(D-1) points, into `dst`
-__device__ void interp_side(const Args& a, const Side& sd, int64_t f0, int nf, double* A, double* Bb, double* dst)
+__device__ __forceinline__ void interp_n(int D, int n, int q, int S, const double* B, const double* bf, const double* x,
+                                         int axis, const int64_t* elem, const int* off, int nf, double* A, double* Bb,
+                                         double* dst)
 {
-  const int T = a.D - 1, n = sd.n, q = a.q, S = a.S;
-  const int nt = ipow(n, T), nd = nt * n;
+  const int T = D - 1;
+  const int nt = ipow(n, T), nd = nt * n, ax = ipow(n, axis);
   double* first = T == 0 ? dst : A;
   for (int t = threadIdx.x; t < nf * nt; t += kThreads) {
     const int f = t / nt, tt = t - f * nt;
-    const int64_t e = side_elem(a, sd, f0 + f, nullptr);
-    const double* u = sd.x + e * nd;
+    const double* u = x + elem[f] * nd + off[tt];
     double s = 0.0;
     for (int i = 0; i < n; ++i) {
-      const double w = sd.bf[i];
-      if (w != 0.0) s += w * u[dof_index(a.D, a.axis, n, tt, i)];
+      const double w = bf[i];
+      if (w != 0.0) s += w * u[i * ax];
     }
     first[f * S + tt] = s;
   }
@@ -135,18 +146,24 @@ __device__ void interp_side(const Args& a, const Side& sd, int64_t f0, int nf, d
   const double* src = A;
   for (int k = 0; k < T; ++k) {
     double* out = k == T - 1 ? dst : (src == A ? Bb : A);
-    sweep<false>(sd.B, n, n, q, ipow(q, k), ipow(n, T - 1 - k), nf, S, src, out);
+    sweep<false>(B, n, n, q, ipow(q, k), ipow(n, T - 1 - k), nf, S, src, out);
     __syncthreads();
     src = out;
   }
 }
 
+__device__ void interp_side(const Args& a, const Side& sd, int axis, const int64_t* elem, const int* off, int nf,
+                            double* A, double* Bb, double* dst)
+{
+  interp_n(a.D, sd.n, a.q, a.S, sd.B, sd.bf, sd.x, axis, elem, off, nf, A, Bb, dst);
+}
+
 // B^T of the flux (in F) spread back along the normal: y += sgn * bf (x) B^T F
-__device__ void spread_side(const Args& a, const Side& sd, int64_t f0, int nf, const double* F, double* A, double* Bb,
-                            double sgn)
+__device__ void spread_side(const Args& a, const Side& sd, const int64_t* elem, const int* off, int nf, const double* F,
+                            double* A, double* Bb, double sgn)
 {
   const int T = a.D - 1, n = sd.n, q = a.q, S = a.S;
-  const int nt = ipow(n, T), nd = nt * n;
+  const int nt = ipow(n, T), nd = nt * n, ax = ipow(n, a.axis);
   const double* src = F;
   for (int k = 0; k < T; ++k) {
     double* out = (k % 2 == 0) ? A : Bb;
@@ -159,8 +176,7 @@ __device__ void spread_side(const Args& a, const Side& sd, int64_t f0, int nf, c
     const int i = r / nt, tt = r - i * nt;
     const double w = sd.bf[i];
     if (w == 0.0) continue;
-    const int64_t e = side_elem(a, sd, f0 + f, nullptr);
-    double* yy = sd.y + e * nd + dof_index(a.D, a.axis, n, tt, i);
+    double* yy = sd.y + elem[f] * nd + off[tt] + i * ax;
     *yy += sgn * (w * src[f * S + tt]);
   }
   __syncthreads();
@@ -193,25 +209,34 @@ __device__ __forceinline__ double beta_axis(const Params<double>& p, int c, cons
 __global__ void __launch_bounds__(kThreads) k_faces(Args a)
 {
   extern __shared__ double lds[];
+  __shared__ Idx ix;
+  __shared__ int sub0[kMaxFpb][kMaxDim];  // minus cell indices per axis
   const int64_t f0 = (int64_t)blockIdx.x * a.fpb;
   const int nf = (int)min((int64_t)a.fpb, a.nfaces - f0);
-  const int S = a.S;
+  const int S = a.S, T = a.D - 1, q = a.q;
+  for (int t = threadIdx.x; t < nf; t += kThreads) {
+    ix.elem[0][t] = side_elem(a, a.s[0], f0 + t, sub0[t]);
+    ix.elem[1][t] = side_elem(a, a.s[1], f0 + t, nullptr);
+  }
+  for (int s = 0; s < 2; ++s) {
+    const int n = a.s[s].n, nt = ipow(n, T);
+    for (int t = threadIdx.x; t < nt; t += kThreads) ix.off[s][t] = dof_index(a.D, a.axis, n, t, 0);
+  }
+  __syncthreads();
   double* A = lds;
   double* Bb = lds + a.fpb * S;
   double* Um = lds + 2 * a.fpb * S;
   double* Up = lds + 3 * a.fpb * S;
-  interp_side(a, a.s[0], f0, nf, A, Bb, Um);
-  interp_side(a, a.s[1], f0, nf, A, Bb, Up);
+  interp_side(a, a.s[0], a.axis, ix.elem[0], ix.off[0], nf, A, Bb, Um);
+  interp_side(a, a.s[1], a.axis, ix.elem[1], ix.off[1], nf, A, Bb, Up);
   // upwind flux at the face points of the minus cell (:512-540)
-  const int T = a.D - 1, q = a.q;
   const int nq = ipow(q, T);
   const Params<double>& p = a.geo;
   const double scale = (double)a.sign / (double)p.h[a.axis];
   for (int t = threadIdx.x; t < nf * nq; t += kThreads) {
     const int f = t / nq;
     int r = t - f * nq;
-    int sub[kMaxDim], k[kMaxDim];
-    (void)side_elem(a, a.s[0], f0 + f, sub);
+    int k[kMaxDim];
     double w = 1.0;
     for (int d = 0; d < a.D; ++d) {
       if (d == a.axis) {
@@ -222,14 +247,139 @@ __global__ void __launch_bounds__(kThreads) k_faces(Args a)
         w *= p.tab.w[k[d]];
       }
     }
-    const double dot = beta_axis(p, a.axis, sub, k, q, a.D) * scale;
+    const double dot = beta_axis(p, a.axis, sub0[f], k, q, a.D) * scale;
     const int at = f * S + (t - f * nq);
     const double up = dot > 0.0 ? dot * Um[at] : dot * Up[at];
     Um[at] = w * p.detJ * up;
   }
   __syncthreads();
-  spread_side(a, a.s[0], f0, nf, Um, A, Bb, 1.0);
-  spread_side(a, a.s[1], f0, nf, Um, A, Bb, -1.0);
+  spread_side(a, a.s[0], ix.elem[0], ix.off[0], nf, Um, A, Bb, 1.0);
+  spread_side(a, a.s[1], ix.elem[1], ix.off[1], nf, Um, A, Bb, -1.0);
+}
+
+// Interior faces with one block per run of FPB consecutive elements: the
+// block visits each of its elements' 2D faces (axis by axis, lower face then
+// upper face), forms that face's flux exactly as above (minus = the upper
+// cell, its geometry and field) and adds only its own share, + for the lower
+// face (the element is the minus side) and - for the upper face (the element
+// is the plus side), into an LDS accumulator; the block ends with one
+// coalesced y += acc.  Each flux is formed twice, once per side, with the
+// same arithmetic, so both sides see the same value; y is read and written
+// once per apply instead of once per axis and colour.
+// DD, NN, QQ > 0 fix the dimension, nodes and points at compile time (the
+// loops unroll and the index arithmetic folds); 0 reads them from `a`.
+constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+
+template <int DD, int NN, int QQ>
+__global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
+{
+  extern __shared__ double lds[];
+  __shared__ Idx ix;
+  __shared__ int sub[kMaxFpb][kMaxDim];   // owner cell indices
+  __shared__ int subm[kMaxFpb][kMaxDim];  // the current face's minus cell
+  __shared__ int ok[kMaxFpb];             // the face is interior
+  const Side& sd = a.s[0];
+  const int64_t E = a.nfaces;  // elements
+  const int64_t e0 = (int64_t)blockIdx.x * a.fpb;
+  const int nf = (int)min((int64_t)a.fpb, E - e0);
+  const int D = DD > 0 ? DD : a.D;
+  const int n = NN > 0 ? NN : sd.n;
+  const int q = QQ > 0 ? QQ : a.q;
+  const int S = DD > 0 ? cpow(NN > QQ ? NN : QQ, DD - 1) : a.S;
+  const int T = D - 1;
+  const int nt = ipow(n, T), nd = nt * n, nq = ipow(q, T);
+  double* A = lds;
+  double* Bb = A + a.fpb * S;
+  double* Um = Bb + a.fpb * S;
+  double* Up = Um + a.fpb * S;
+  double* acc = Up + a.fpb * S;
+  for (int t = threadIdx.x; t < nf; t += kThreads) {
+    int64_t r = e0 + t;
+    for (int d = 0; d < D; ++d) {
+      sub[t][d] = (int)(r % sd.M[d]);
+      r /= sd.M[d];
+    }
+  }
+  for (int t = threadIdx.x; t < nf * nd; t += kThreads) acc[t] = 0.0;
+  const Params<double>& p = a.geo;
+  int64_t str = 1;
+  for (int axis = 0; axis < D; ++axis) {
+    const double scale = -1.0 / (double)p.h[axis];  // minus normal -e_axis
+    for (int slot = 0; slot < 2; ++slot) {
+      __syncthreads();  // the previous face's tables are no longer read
+      for (int t = threadIdx.x; t < nt; t += kThreads) ix.off[0][t] = dof_index(D, axis, n, t, 0);
+      for (int t = threadIdx.x; t < nf; t += kThreads) {
+        const int64_t e = e0 + t;
+        const bool in = slot == 0 ? sub[t][axis] > 0 : sub[t][axis] < sd.M[axis] - 1;
+        ok[t] = in;
+        ix.elem[0][t] = in && slot == 1 ? e + str : e;  // minus: the upper cell
+        ix.elem[1][t] = in && slot == 0 ? e - str : e;  // plus: the lower cell
+        for (int d = 0; d < D; ++d) subm[t][d] = sub[t][d] + ((in && slot == 1 && d == axis) ? 1 : 0);
+      }
+      __syncthreads();
+      interp_n(D, n, q, S, a.s[0].B, a.s[0].bf, a.s[0].x, axis, ix.elem[0], ix.off[0], nf, A, Bb, Um);
+      interp_n(D, n, q, S, a.s[1].B, a.s[1].bf, a.s[1].x, axis, ix.elem[1], ix.off[0], nf, A, Bb, Up);
+      for (int t = threadIdx.x; t < nf * nq; t += kThreads) {
+        const int f = t / nq;
+        const int at = f * S + (t - f * nq);
+        if (!ok[f]) {
+          Um[at] = 0.0;
+          continue;
+        }
+        int r = t - f * nq;
+        int k[kMaxDim];
+        double w = 1.0;
+        for (int d = 0; d < D; ++d) {
+          if (d == axis) {
+            k[d] = q;  // xi = 0 of the minus cell
+          } else {
+            k[d] = r % q;
+            r /= q;
+            w *= p.tab.w[k[d]];
+          }
+        }
+        const double dot = beta_axis(p, axis, subm[f], k, q, D) * scale;
+        const double up = dot > 0.0 ? dot * Um[at] : dot * Up[at];
+        Um[at] = w * p.detJ * up;
+      }
+      __syncthreads();
+      // the owner's share: + b0 (x) B^T F on its lower face, - b1 (x) B^T F on its upper face
+      const Side& own = a.s[slot];
+      const double sgn = slot == 0 ? 1.0 : -1.0;
+      const int ax = ipow(n, axis);
+      const double* src = Um;
+      for (int kk = 0; kk < T; ++kk) {
+        double* out = (kk % 2 == 0) ? A : Bb;
+        sweep<true>(own.B, n, q, n, ipow(n, kk), ipow(q, T - 1 - kk), nf, S, src, out);
+        __syncthreads();
+        src = out;
+      }
+      for (int t = threadIdx.x; t < nf * nd; t += kThreads) {
+        const int f = t / nd, r = t - f * nd;
+        const int i = r / nt, tt = r - i * nt;
+        const double w = own.bf[i];
+        if (w == 0.0 || !ok[f]) continue;
+        acc[f * nd + ix.off[0][tt] + i * ax] += sgn * (w * src[f * S + tt]);
+      }
+    }
+    str *= sd.M[axis];
+  }
+  __syncthreads();
+  double* y = sd.y + e0 * nd;
+  for (int t = threadIdx.x; t < nf * nd; t += kThreads) y[t] += acc[t];
+}
+
+using OwnerKernel = void (*)(Args);
+
+OwnerKernel owner_kernel(int D, int n, int q)
+{
+#define GD_FACE_OWNER(d, nn, qq) \
+  if (D == d && n == nn && q == qq) return k_faces_owner<d, nn, qq>;
+  GD_FACE_OWNER(3, 2, 2) GD_FACE_OWNER(3, 2, 3) GD_FACE_OWNER(3, 3, 3) GD_FACE_OWNER(3, 3, 4)
+  GD_FACE_OWNER(3, 4, 4) GD_FACE_OWNER(3, 4, 5) GD_FACE_OWNER(3, 5, 6) GD_FACE_OWNER(2, 3, 3)
+  GD_FACE_OWNER(2, 4, 4) GD_FACE_OWNER(6, 2, 2) GD_FACE_OWNER(6, 2, 3)
+#undef GD_FACE_OWNER
+  return k_faces_owner<0, 0, 0>;
 }
 
 int side_setup(const gd_plan* pl, int q, int xi, const double* x, double* y, Side& s)
@@ -285,7 +435,7 @@ int common_args(const gd_plan* m, const gd_plan* p, int axis, int sign, Args& a)
   a.q = q;
   a.sign = sign;
   a.S = (int)S;
-  a.fpb = std::max(1, std::min(64, (int)(kLdsBytes / (4 * S * (long long)sizeof(double)))));
+  a.fpb = std::max(1, std::min(kMaxFpb, (int)(kLdsBytes / (4 * S * (long long)sizeof(double)))));
   a.xi_minus = sign > 0 ? 1 : 0;
   const TableSet* ts = find_tables(m, m->desc.order + 1, q);
   if (!ts) return fail(GD_ERR_UNSUPPORTED, "face operator: plan has no tables for its rule");
@@ -303,6 +453,27 @@ extern "C" int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, doubl
   if (plan->ndofs > 0 && (!x || !y)) return gd::fail(GD_ERR_INVALID, "gd_adv_face_apply_f64: null vector");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int D = plan->dim;
+  {
+    Args a;
+    if (int rc = common_args(plan, plan, 0, -1, a)) return rc;
+    const int n = plan->desc.order + 1;
+    long long nd = 1;
+    for (int d = 0; d < D; ++d) nd *= n;
+    const long long per = (4LL * a.S + nd) * (long long)sizeof(double);
+    const int fpb = (int)std::min<long long>(kMaxFpb, kLdsBytes / per);
+    if (fpb >= 8) {
+      // one launch: every element adds its own share of each of its faces
+      a.fpb = fpb;
+      a.nfaces = plan->E;
+      if (int rc = side_setup(plan, a.q, 0, x, y, a.s[0])) return rc;
+      if (int rc = side_setup(plan, a.q, 1, x, y, a.s[1])) return rc;
+      if (a.nfaces <= 0) return GD_OK;
+      const int blocks = (int)((a.nfaces + fpb - 1) / fpb);
+      hipLaunchKernelGGL(owner_kernel(D, n, a.q), dim3((unsigned)blocks), dim3(kThreads), (size_t)fpb * per, st, a);
+      const hipError_t e = hipGetLastError();
+      return e == hipSuccess ? GD_OK : gd::hip_fail(e, "face kernel launch");
+    }
+  }
   for (int axis = 0; axis < D; ++axis) {
     // interior faces of this axis, minus cell = the upper one (reference
     // normal -e_axis), in two colours by the parity of its index

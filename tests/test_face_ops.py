@@ -206,3 +206,29 @@ def test_face_gpu_rejects_unsupported_inputs():
     v = torch.zeros(fm.num_dofs + fp.num_dofs, dtype=torch.float64, device="cuda")
     with pytest.raises(G._lib.GdError, match="same rule"):
         bad(v, v.clone())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize(
+    "D,p,q,n,coef",
+    [
+        (3, 1, 2, 4, "c2"),    # one launch, sizes fixed at compile time
+        (3, 4, 5, 3, "trig"),  # one launch, sizes read at run time
+        (6, 2, 3, 2, "phase"),  # faces too large for one launch: colours per axis
+        (1, 3, 4, 7, "const"),
+    ],
+)
+def test_face_gpu_kernel_paths_against_oracle(D, p, q, n, coef):
+    import torch
+    from cases import beta_field
+
+    fes = G.MakeFiniteElementSpace(G.CartesianMesh.unit(D, n, False), G.MakeLobattoFiniteElement(p))
+    op = G.MakeAdvectionFaceOperator(fes, G.MakeCartesianInteriorFaceConnectivity(fes.mesh.sizes),
+                                     G.MakeIntegrationRule(q), beta_field(coef, D))
+    x = G.fill_random(fes.num_dofs, 23)
+    y0 = G.fill_random(fes.num_dofs, 24)
+    y = _dev(y0)
+    op(_dev(x), y)
+    ref = np.zeros_like(x)
+    oracle.face_operator([n] * D, [1.0 / n] * D, [0.0] * D, p + 1, 0, q, beta_numpy(coef, D), x, ref)
+    assert rel_l2(y.cpu().numpy() - y0, ref) <= F64_TOL
