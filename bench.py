@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES comparison")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-padapt", action="store_true", help="skip the 6D p-adaptive apply")
+    ap.add_argument("--face-ops-only", action="store_true",
+                    help="only time the face-centric operators (one JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time a plain device copy of the apply's input/output bytes (counter calibration)")
     return ap.parse_args()
@@ -353,6 +355,78 @@ def adjoint_48(steps, warmup, n=48, p=4, q=5):
     return res
 
 
+def face_ops(steps, warmup, n=64, p=3):
+    """GenDiL's face-centric advection operators on device-resident vectors:
+    the interior faces of a 64^3 non-periodic mesh (GLL p=3, 4 Gauss points,
+    the benchmark field) and the +x interface of two 64^3 meshes of orders 1
+    and 3 sharing 5 points (beta = (y, z, x), the p-adaptivity test's field).
+    Both accumulate into y, so the algorithmic bytes are x read, y read and y
+    written once: 24 B per DoF."""
+    import numpy as np
+    import torch
+
+    import gendil_amd as G
+    from gendil_amd import workloads as W
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    out = {}
+    fes = G.MakeFiniteElementSpace(G.CartesianMesh.unit(3, n, False), G.MakeLobattoFiniteElement(p))
+    op = G.MakeAdvectionFaceOperator(fes, G.MakeCartesianInteriorFaceConnectivity(fes.mesh.sizes),
+                                     G.MakeIntegrationRule(p + 1), W.advection_c2())
+    N = fes.num_dofs
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(N, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    y = torch.zeros_like(x)
+    ms = timed(lambda: op(x, y))
+    out["interior"] = {
+        "workload": f"interior faces, 3D {n}^3 non-periodic, GLL p={p}, {p + 1} GL points, benchmark field, FP64",
+        "dofs": N, "ms": ms, "value": N / (ms * 1e-3) / 1e9, "unit": "GDoF/s",
+        "roofline_frac": 24.0 * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    del x, y
+    try:
+        import oracle
+
+        if oracle.ref_available():
+            threads = min(16, os.cpu_count() or 1)
+            r = oracle.ref_bench("advface", 3, p, p + 1, "L", n, False, "c2", "none", 12345, 1,
+                                 env={"OMP_NUM_THREADS": str(threads)})
+            out["interior"]["cpu_reference"] = {"value": r["dofs_per_s"] / 1e9, "unit": "GDoF/s", "cores": threads,
+                                                "kind": "reference"}
+    except Exception as e:  # the GPU figure stands on its own
+        out["interior"]["cpu_reference"] = {"error": str(e)[:200]}
+    h = 1.0 / n
+    fm = G.MakeFiniteElementSpace(G.Cartesian3DMesh(h, n, n, n), G.MakeLobattoFiniteElement(1))
+    fp = G.MakeFiniteElementSpace(G.Cartesian3DMesh(h, n, n, n, origin=(1.0, 0.0, 0.0)),
+                                  G.MakeLobattoFiniteElement(3), G.L2Restriction(fm.num_dofs))
+    yzx = G.AffineField(np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0]]), np.zeros(3))
+    im = G.MakeAdvectionFaceOperator(fm, fp, G.CartesianIntermeshFaceConnectivity(3, 3, (n, n, n), (n, n, n)),
+                                     G.MakeIntegrationRule(5), yzx)
+    M = fm.num_dofs + fp.num_dofs
+    x = torch.rand(M, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    y = torch.zeros_like(x)
+    ms = timed(lambda: im(x, y))
+    faces = n * n
+    out["intermesh"] = {
+        "workload": f"+x interface of two {n}^3 meshes, GLL p=1 | p=3, 5 shared GL points, beta=(y,z,x), FP64",
+        "faces": faces, "ms": ms, "faces_per_s": faces / (ms * 1e-3),
+    }
+    del x, y
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -365,6 +439,9 @@ def main():
     import gendil_amd as G
     from gendil_amd import workloads as W
 
+    if args.face_ops_only:
+        print(json.dumps({"face_ops": face_ops(args.steps, args.warmup)}))
+        return
     n = args.n
     mesh = G.CartesianMesh.unit(3, n, True)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
@@ -433,6 +510,8 @@ def main():
         result["product6d"] = product_6d(max(20, args.steps // 2), 5)
         torch.cuda.empty_cache()
         result["adjoint48"] = adjoint_48(max(10, args.steps // 10), 2)
+        torch.cuda.empty_cache()
+        result["face_ops"] = face_ops(max(10, args.steps // 10), 2)
         torch.cuda.empty_cache()
     if not args.no_solve:
         result["solve"] = solve_comparison()
