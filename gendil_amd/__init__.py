@@ -68,6 +68,7 @@ def __getattr__(name):
         "AdvectionNonconformingFaceOperator",
         "CartesianInteriorFaceConnectivity",
         "CartesianIntermeshFaceConnectivity",
+        "NonconformingCartesianIntermeshFaceConnectivity",
         "MakeCartesianInteriorFaceConnectivity",
         "MakeAdvectionFaceOperator",
     ):

@@ -108,6 +108,7 @@ SIGNATURES = {
     "gd_adv_grad_beta_f64": (C.c_int, [_p, _d, _p, _p, _p, _ppd, _p]),
     "gd_adv_face_apply_f64": (C.c_int, [_p, _p, _p, _p]),
     "gd_adv_intermesh_apply_f64": (C.c_int, [_p, _p, C.c_int32, _p, _p, _p, _p, _p]),
+    "gd_adv_hmesh_apply_f64": (C.c_int, [_p, _p, C.c_int32, _p, _p, _p, _p, _p]),
     "gd_plan_autotune": (C.c_int, [_p, _i32, _d, _d, _i32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "gd_plan_set_variant": (C.c_int, [_p, _i32, _i32]),
     "gd_plan_query_variant": (C.c_int, [_p, _i32, _d, _d, C.POINTER(C.c_int32)]),

@@ -42,6 +42,13 @@ struct Side {
   int64_t M[kMaxDim];
   int n;
   int64_t base, stride;  // cell index along the axis = base + stride * face row
+  // tangential cell index = face grid index * mul + add (1, 0: the face grid
+  // is this side's own grid)
+  int mul[kMaxDim], add[kMaxDim];
+  // subface matrices of a coarse side: [.][q][n] blocks, the one for axis d
+  // at block bsel[d]; nullptr: B
+  const double* Bmap;
+  int bsel[kMaxDim];
 };
 
 struct Args {
@@ -51,6 +58,7 @@ struct Args {
   Side s[2];           // 0 minus, 1 plus
   Params<double> geo;  // minus plan: origin, h, detJ, field, rule tables
   int xi_minus;        // 0: the minus face is at xi = 0, 1: at xi = 1
+  double wscale;       // subface weight factor prod 1 / r (1: conforming)
 };
 
 __device__ __forceinline__ int ipow(int b, int e)
@@ -67,7 +75,7 @@ __device__ __forceinline__ int64_t side_elem(const Args& a, const Side& sd, int6
   for (int d = 0; d < a.D; ++d) {
     const int64_t c = r % a.G[d];
     r /= a.G[d];
-    const int64_t s = d == a.axis ? sd.base + sd.stride * c : c;
+    const int64_t s = d == a.axis ? sd.base + sd.stride * c : c * sd.mul[d] + sd.add[d];
     if (sub) sub[d] = (int)s;
     lin += s * str;
     str *= sd.M[d];
@@ -98,9 +106,9 @@ __device__ __forceinline__ int dof_index(int D, int axis, int n, int t, int i)
 // in [outer][a][inner] -> out [outer][b][inner], out = M in, M(j, i) =
 // TRANS ? B[i][j] : B[j][i] with B row-major [q][n]
 template <bool TRANS>
-__device__ __forceinline__ void sweep(const double* B, int n, int a, int b, int inner, int outer, int nf, int S,
+__device__ __forceinline__ void sweep(const double* B0, int n, int a, int b, int inner, int outer, int nf, int S,
                                       const double* in,
-                      double* out)
+                      double* out, const int* foff = nullptr)
 {
   const int per = outer * b * inner;
   for (int t = threadIdx.x; t < nf * per; t += kThreads) {
@@ -110,6 +118,7 @@ __device__ __forceinline__ void sweep(const double* B, int n, int a, int b, int 
     r -= o * b * inner;
     const int j = r / inner, c = r - j * inner;
     const double* src = in + f * S + o * a * inner + c;
+    const double* B = foff ? B0 + foff[f * kMaxDim] : B0;  // per-face matrix (subfaces)
     double s = 0.0;
     for (int i = 0; i < a; ++i) s += (TRANS ? B[i * n + j] : B[j * n + i]) * src[i * inner];
     out[f * S + o * b * inner + j * inner + c] = s;
@@ -127,7 +136,7 @@ struct Idx {
 // the side's trace interpolated to the q^(D-1) points, into `dst`
 __device__ __forceinline__ void interp_n(int D, int n, int q, int S, const double* B, const double* bf, const double* x,
                                          int axis, const int64_t* elem, const int* off, int nf, double* A, double* Bb,
-                                         double* dst)
+                                         double* dst, const int* foff = nullptr)
 {
   const int T = D - 1;
   const int nt = ipow(n, T), nd = nt * n, ax = ipow(n, axis);
@@ -146,28 +155,29 @@ __device__ __forceinline__ void interp_n(int D, int n, int q, int S, const doubl
   const double* src = A;
   for (int k = 0; k < T; ++k) {
     double* out = k == T - 1 ? dst : (src == A ? Bb : A);
-    sweep<false>(B, n, n, q, ipow(q, k), ipow(n, T - 1 - k), nf, S, src, out);
+    sweep<false>(B, n, n, q, ipow(q, k), ipow(n, T - 1 - k), nf, S, src, out, foff ? foff + k : nullptr);
     __syncthreads();
     src = out;
   }
 }
 
 __device__ void interp_side(const Args& a, const Side& sd, int axis, const int64_t* elem, const int* off, int nf,
-                            double* A, double* Bb, double* dst)
+                            double* A, double* Bb, double* dst, const int* foff = nullptr)
 {
-  interp_n(a.D, sd.n, a.q, a.S, sd.B, sd.bf, sd.x, axis, elem, off, nf, A, Bb, dst);
+  interp_n(a.D, sd.n, a.q, a.S, foff ? sd.Bmap : sd.B, sd.bf, sd.x, axis, elem, off, nf, A, Bb, dst, foff);
 }
 
 // B^T of the flux (in F) spread back along the normal: y += sgn * bf (x) B^T F
 __device__ void spread_side(const Args& a, const Side& sd, const int64_t* elem, const int* off, int nf, const double* F,
-                            double* A, double* Bb, double sgn)
+                            double* A, double* Bb, double sgn, const int* foff = nullptr)
 {
   const int T = a.D - 1, n = sd.n, q = a.q, S = a.S;
   const int nt = ipow(n, T), nd = nt * n, ax = ipow(n, a.axis);
   const double* src = F;
   for (int k = 0; k < T; ++k) {
     double* out = (k % 2 == 0) ? A : Bb;
-    sweep<true>(sd.B, n, q, n, ipow(n, k), ipow(q, T - 1 - k), nf, S, src, out);
+    sweep<true>(foff ? sd.Bmap : sd.B, n, q, n, ipow(n, k), ipow(q, T - 1 - k), nf, S, src, out,
+                foff ? foff + k : nullptr);
     __syncthreads();
     src = out;
   }
@@ -214,9 +224,16 @@ __global__ void __launch_bounds__(kThreads) k_faces(Args a)
   const int64_t f0 = (int64_t)blockIdx.x * a.fpb;
   const int nf = (int)min((int64_t)a.fpb, a.nfaces - f0);
   const int S = a.S, T = a.D - 1, q = a.q;
+  __shared__ int boff[kMaxFpb][kMaxDim];   // the minus side's subface matrix per tangential axis
+  const bool mapped = a.s[0].Bmap != nullptr;
   for (int t = threadIdx.x; t < nf; t += kThreads) {
     ix.elem[0][t] = side_elem(a, a.s[0], f0 + t, sub0[t]);
     ix.elem[1][t] = side_elem(a, a.s[1], f0 + t, nullptr);
+    if (mapped) {
+      int k = 0;
+      for (int d = 0; d < a.D; ++d)
+        if (d != a.axis) boff[t][k++] = a.s[0].bsel[d] * a.q * a.s[0].n;
+    }
   }
   for (int s = 0; s < 2; ++s) {
     const int n = a.s[s].n, nt = ipow(n, T);
@@ -227,7 +244,7 @@ __global__ void __launch_bounds__(kThreads) k_faces(Args a)
   double* Bb = lds + a.fpb * S;
   double* Um = lds + 2 * a.fpb * S;
   double* Up = lds + 3 * a.fpb * S;
-  interp_side(a, a.s[0], a.axis, ix.elem[0], ix.off[0], nf, A, Bb, Um);
+  interp_side(a, a.s[0], a.axis, ix.elem[0], ix.off[0], nf, A, Bb, Um, mapped ? &boff[0][0] : nullptr);
   interp_side(a, a.s[1], a.axis, ix.elem[1], ix.off[1], nf, A, Bb, Up);
   // upwind flux at the face points of the minus cell (:512-540)
   const int nq = ipow(q, T);
@@ -250,10 +267,10 @@ __global__ void __launch_bounds__(kThreads) k_faces(Args a)
     const double dot = beta_axis(p, a.axis, sub0[f], k, q, a.D) * scale;
     const int at = f * S + (t - f * nq);
     const double up = dot > 0.0 ? dot * Um[at] : dot * Up[at];
-    Um[at] = w * p.detJ * up;
+    Um[at] = (w * a.wscale) * p.detJ * up;
   }
   __syncthreads();
-  spread_side(a, a.s[0], ix.elem[0], ix.off[0], nf, Um, A, Bb, 1.0);
+  spread_side(a, a.s[0], ix.elem[0], ix.off[0], nf, Um, A, Bb, 1.0, mapped ? &boff[0][0] : nullptr);
   spread_side(a, a.s[1], ix.elem[1], ix.off[1], nf, Um, A, Bb, -1.0);
 }
 
@@ -391,7 +408,13 @@ int side_setup(const gd_plan* pl, int q, int xi, const double* x, double* y, Sid
   s.y = y;
   s.B = ts->t64.B;
   s.bf = xi ? ts->t64.b1 : ts->t64.b0;
-  for (int d = 0; d < kMaxDim; ++d) s.M[d] = d < pl->dim ? pl->desc.sizes[d] : 1;
+  for (int d = 0; d < kMaxDim; ++d) {
+    s.M[d] = d < pl->dim ? pl->desc.sizes[d] : 1;
+    s.mul[d] = 1;
+    s.add[d] = 0;
+    s.bsel[d] = 0;
+  }
+  s.Bmap = nullptr;
   s.n = n;
   return GD_OK;
 }
@@ -413,7 +436,7 @@ int launch(Args& a, hipStream_t st)
   return e == hipSuccess ? GD_OK : hip_fail(e, "face kernel launch");
 }
 
-int common_args(const gd_plan* m, const gd_plan* p, int axis, int sign, Args& a)
+int common_args(const gd_plan* m, const gd_plan* p, int axis, int sign, Args& a, bool refined = false)
 {
   const int D = m->dim;
   if (p->dim != D) return fail(GD_ERR_INVALID, "face operator: plans differ in dimension");
@@ -422,7 +445,7 @@ int common_args(const gd_plan* m, const gd_plan* p, int axis, int sign, Args& a)
     return fail(GD_ERR_UNSUPPORTED, "face operator: needs a separable or affine advection field");
   const int q = m->desc.num_quad;
   if (p->desc.num_quad != q) return fail(GD_ERR_INVALID, "face operator: both plans must use the same rule");
-  for (int d = 0; d < D; ++d)
+  for (int d = 0; d < D && !refined; ++d)
     if (d != axis && m->desc.sizes[d] != p->desc.sizes[d])
       return fail(GD_ERR_INVALID, "face operator: tangential mesh sizes differ");
   const int nmax = std::max({m->desc.order + 1, p->desc.order + 1, q});
@@ -437,6 +460,7 @@ int common_args(const gd_plan* m, const gd_plan* p, int axis, int sign, Args& a)
   a.S = (int)S;
   a.fpb = std::max(1, std::min(kMaxFpb, (int)(kLdsBytes / (4 * S * (long long)sizeof(double)))));
   a.xi_minus = sign > 0 ? 1 : 0;
+  a.wscale = 1.0;
   const TableSet* ts = find_tables(m, m->desc.order + 1, q);
   if (!ts) return fail(GD_ERR_UNSUPPORTED, "face operator: plan has no tables for its rule");
   a.geo = make_params<double>(m, ts, 0.0, 1.0);
@@ -526,4 +550,121 @@ extern "C" int gd_adv_intermesh_apply_f64(const gd_plan* minus, const gd_plan* p
   a.s[1].base = sign > 0 ? 0 : plus->desc.sizes[axis] - 1;
   a.s[0].stride = a.s[1].stride = 0;
   return launch(a, static_cast<hipStream_t>(stream));
+}
+
+namespace gd {
+namespace face {
+
+// l_i(x) on the plan's nodes
+double lagrange(const std::vector<double>& xn, int i, double x)
+{
+  double v = 1.0;
+  for (size_t j = 0; j < xn.size(); ++j)
+    if ((int)j != i) v *= (x - xn[j]) / (xn[i] - xn[j]);
+  return v;
+}
+
+// the coarse side's basis at the mapped points (o + xi) / r of every subface
+// offset o < r of every tangential axis, cached on the plan
+int subface_tables(const gd_plan* c, int axis, const int* r, int q, Side& sd, int* bbase)
+{
+  const int D = c->dim, n = c->desc.order + 1;
+  std::vector<int64_t> key = {n, q, axis};
+  for (int d = 0; d < D; ++d) key.push_back(r[d]);
+  const TableSet* ts = find_tables(c, n, q);
+  if (!ts) return fail(GD_ERR_UNSUPPORTED, "face operator: plan has no tables for the shared rule");
+  int nb = 0;
+  for (int d = 0; d < D; ++d) {
+    bbase[d] = nb;
+    if (d != axis) nb += r[d];
+  }
+  if (c->hmap_key != key) {
+    std::vector<double> xn(n), wn(n);
+    if (int rc = gd_rule_points(c->desc.basis, n, xn.data(), wn.data())) return rc;
+    std::vector<double> tab((size_t)nb * q * n);
+    for (int d = 0; d < D; ++d) {
+      if (d == axis) continue;
+      for (int o = 0; o < r[d]; ++o)
+        for (int k = 0; k < q; ++k)
+          for (int i = 0; i < n; ++i)
+            tab[((size_t)(bbase[d] + o) * q + k) * n + i] = lagrange(xn, i, (o + ts->xq[k]) / r[d]);
+    }
+    if (c->hmap_dev) (void)hipFree(c->hmap_dev);
+    c->hmap_dev = nullptr;
+    c->hmap_key.clear();
+    if (hipMalloc(&c->hmap_dev, tab.size() * sizeof(double)) != hipSuccess)
+      return fail(GD_ERR_ALLOC, "face operator: subface tables");
+    if (hipMemcpy(c->hmap_dev, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(GD_ERR_HIP, "face operator: upload of subface tables");
+    c->hmap_key = key;
+  }
+  sd.Bmap = static_cast<const double*>(c->hmap_dev);
+  return GD_OK;
+}
+
+}  // namespace face
+}  // namespace gd
+
+extern "C" int gd_adv_hmesh_apply_f64(const gd_plan* coarse, const gd_plan* fine, int32_t face,
+                                      const double* x_coarse, const double* x_fine, double* y_coarse,
+                                      double* y_fine, gd_stream stream)
+{
+  using namespace gd::face;
+  if (int rc = check_plan(coarse, "gd_adv_hmesh_apply_f64")) return rc;
+  if (int rc = check_plan(fine, "gd_adv_hmesh_apply_f64")) return rc;
+  const int D = coarse->dim;
+  if (fine->dim != D) return gd::fail(GD_ERR_INVALID, "gd_adv_hmesh_apply_f64: plans differ in dimension");
+  if (face < 0 || face >= 2 * D) return gd::fail(GD_ERR_INVALID, "gd_adv_hmesh_apply_f64: face out of range");
+  if (!x_coarse || !x_fine || !y_coarse || !y_fine)
+    return gd::fail(GD_ERR_INVALID, "gd_adv_hmesh_apply_f64: null vector");
+  const int axis = face % D, sign = face >= D ? 1 : -1;
+  int r[gd::kMaxDim];
+  double wscale = 1.0;
+  for (int d = 0; d < gd::kMaxDim; ++d) r[d] = 1;
+  for (int d = 0; d < D; ++d) {
+    if (d == axis) continue;
+    const int64_t mc = coarse->desc.sizes[d], mf = fine->desc.sizes[d];
+    if (mc <= 0 || mf % mc != 0)
+      return gd::fail(GD_ERR_INVALID, "gd_adv_hmesh_apply_f64: fine cells per tangential axis must be a multiple "
+                                      "of the coarse ones");
+    r[d] = (int)(mf / mc);
+    wscale /= (double)r[d];
+  }
+  Args a;
+  if (int rc = common_args(coarse, fine, axis, sign, a, true)) return rc;
+  int64_t nf = 1;
+  for (int d = 0; d < gd::kMaxDim; ++d) {
+    a.G[d] = (d < D && d != axis) ? coarse->desc.sizes[d] : 1;  // one face per coarse cell per launch
+    nf *= a.G[d];
+  }
+  a.nfaces = nf;
+  a.wscale = wscale;
+  if (int rc = side_setup(coarse, a.q, sign > 0 ? 1 : 0, x_coarse, y_coarse, a.s[0])) return rc;
+  if (int rc = side_setup(fine, a.q, sign > 0 ? 0 : 1, x_fine, y_fine, a.s[1])) return rc;
+  int bbase[gd::kMaxDim] = {};
+  if (int rc = subface_tables(coarse, axis, r, a.q, a.s[0], bbase)) return rc;
+  a.s[0].base = sign > 0 ? coarse->desc.sizes[axis] - 1 : 0;
+  a.s[1].base = sign > 0 ? 0 : fine->desc.sizes[axis] - 1;
+  a.s[0].stride = a.s[1].stride = 0;
+  // the r_1 x ... subfaces of a coarse face share its cell: one launch per
+  // subface position, so the faces of a launch touch disjoint cells
+  int o[gd::kMaxDim] = {};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (;;) {
+    for (int d = 0; d < D; ++d) {
+      if (d == axis) continue;
+      a.s[1].mul[d] = r[d];
+      a.s[1].add[d] = o[d];
+      a.s[0].bsel[d] = bbase[d] + o[d];
+    }
+    if (int rc = launch(a, st)) return rc;
+    int d = 0;
+    for (; d < D; ++d) {
+      if (d == axis) continue;
+      if (++o[d] < r[d]) break;
+      o[d] = 0;
+    }
+    if (d == D) break;
+  }
+  return GD_OK;
 }

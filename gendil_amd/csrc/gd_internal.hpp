@@ -145,6 +145,11 @@ struct gd_plan {
   mutable gd::pa::State* pa_state = nullptr;
   // factor tables of the Kronecker-factored 3D kernel (lazily built)
   mutable gd::k3k::State* k3k_state = nullptr;
+  // basis tables at the mapped points of the subfaces of an h-nonconforming
+  // interface (face operators), lazily built for the refinement ratios in
+  // hmap_key
+  mutable void* hmap_dev = nullptr;
+  mutable std::vector<int64_t> hmap_key;
   int coef_version = 0;
   ~gd_plan();
 };

@@ -346,6 +346,7 @@ gd_plan::~gd_plan()
   gd::free_fields(this);
   gd::pa::destroy_state(pa_state);
   gd::k3k::destroy_state(k3k_state);
+  if (hmap_dev) (void)hipFree(hmap_dev);
 }
 
 using gd::fail;

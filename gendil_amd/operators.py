@@ -284,6 +284,23 @@ class CartesianIntermeshFaceConnectivity:
                 raise ValueError("tangential sizes of the two meshes must match")
 
 
+class NonconformingCartesianIntermeshFaceConnectivity(CartesianIntermeshFaceConnectivity):
+    """Face `face` of a coarse mesh (minus) glued to a mesh with an integer
+    number of fine cells per coarse cell along each tangential axis (plus)
+    (nonconformingcartesianintermeshfaceconnectivity.hpp:73-243)."""
+
+    def __init__(self, dim: int, face: int, sizes_coarse, sizes_fine):
+        self.dim, self.face = int(dim), int(face)
+        if not 0 <= self.face < 2 * self.dim:
+            raise ValueError("face out of range")
+        self.sizes_minus = tuple(int(s) for s in sizes_coarse)
+        self.sizes_plus = tuple(int(s) for s in sizes_fine)
+        axis = self.face % self.dim
+        for d in range(self.dim):
+            if d != axis and self.sizes_plus[d] % self.sizes_minus[d] != 0:
+                raise ValueError("fine cells per tangential axis must be a multiple of the coarse ones")
+
+
 class AdvectionFaceOperator:
     """y += the upwind fluxes of every interior face, computed once per face
     (AdvectionFaceOperator, advectionoperator.hpp:1105-1180).  Like GenDiL's,
@@ -315,8 +332,9 @@ class AdvectionNonconformingFaceOperator:
     + on the minus space's part of y, - on the plus space's, accumulated.  The
     two spaces live in one vector through their L2Restriction shifts."""
 
-    def __init__(self, minus: Plan, plus: Plan, face: int):
+    def __init__(self, minus: Plan, plus: Plan, face: int, refined: bool = False):
         self.minus, self.plus, self.face = minus, plus, int(face)
+        self._entry = "gd_adv_hmesh_apply_f64" if refined else "gd_adv_intermesh_apply_f64"
 
     def __call__(self, x: torch.Tensor, y: torch.Tensor):
         fm, fp = self.minus.fes, self.plus.fes
@@ -326,9 +344,9 @@ class AdvectionNonconformingFaceOperator:
                            (yp, fp.num_dofs, "y")):
             _check_vec(t, n, torch.float64, name)
         L.check(
-            L.lib().gd_adv_intermesh_apply_f64(self.minus.handle, self.plus.handle, self.face, xm.data_ptr(),
-                                               xp.data_ptr(), ym.data_ptr(), yp.data_ptr(), _stream()),
-            "gd_adv_intermesh_apply_f64",
+            getattr(L.lib(), self._entry)(self.minus.handle, self.plus.handle, self.face, xm.data_ptr(),
+                                          xp.data_ptr(), ym.data_ptr(), yp.data_ptr(), _stream()),
+            self._entry,
         )
         return y
 
@@ -356,5 +374,6 @@ def MakeAdvectionFaceOperator(*args):
             raise ValueError("face connectivity sizes differ from the meshes")
         pm, pp = Plan(fm, ir), Plan(fp, ir)
         pm.set_beta(adv)
-        return AdvectionNonconformingFaceOperator(pm, pp, conn.face)
+        return AdvectionNonconformingFaceOperator(
+            pm, pp, conn.face, refined=isinstance(conn, NonconformingCartesianIntermeshFaceConnectivity))
     raise TypeError("MakeAdvectionFaceOperator takes 4 or 5 arguments")

@@ -163,6 +163,19 @@ int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, double* y, gd_st
 int gd_adv_intermesh_apply_f64(const gd_plan* minus, const gd_plan* plus, int32_t face,
                                const double* x_minus, const double* x_plus, double* y_minus,
                                double* y_plus, gd_stream stream);
+/* Face `face` of a coarse mesh (minus) glued to a fine mesh (plus) with an
+ * integer number r_d of fine cells per coarse cell along each tangential
+ * axis: one face per fine cell on the interface, the coarse side interpolated
+ * at the subface's mapped points with weights scaled by 1/r_d, geometry and
+ * beta at the coarse face's own rule points, as GenDiL's operator does.
+ * Replaces AdvectionNonconformingFaceOperator over
+ * NonconformingCartesianIntermeshFaceConnectivity<dim, face>
+ * (nonconformingcartesianintermeshfaceconnectivity.hpp:73-243;
+ * advectionoperator.hpp:596-676; facetquaddata.hpp:86-155).  Same rule on
+ * both plans; accumulates. */
+int gd_adv_hmesh_apply_f64(const gd_plan* coarse, const gd_plan* fine, int32_t face,
+                           const double* x_coarse, const double* x_fine, double* y_coarse,
+                           double* y_fine, gd_stream stream);
 
 /* ---- kernel variant selection (autotuner) ------------------------------ */
 /* Times every compiled variant of the apply for this plan on scratch

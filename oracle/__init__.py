@@ -453,3 +453,76 @@ def intermesh_operator(face, q, minus, plus, beta, u_m, u_p, y_m, y_p):
     D = len(minus["sizes"])
     pairs = intermesh_face_pairs(minus["sizes"], plus["sizes"], face)
     advection_faces(face % D, 1 if face >= D else -1, q, minus, plus, pairs, beta, u_m, u_p, y_m, y_p)
+
+
+def hmesh_operator(face, q, coarse, fine, beta, u_c, u_f, y_c, y_f):
+    """GenDiL's nonconforming face operator over
+    NonconformingCartesianIntermeshFaceConnectivity<D, face>
+    (nonconformingcartesianintermeshfaceconnectivity.hpp:115-243, operator
+    advectionoperator.hpp:596-676): face `face` of the coarse (minus) mesh
+    glued to a mesh refined by an integer ratio r_d = fine/coarse cells along
+    each tangential axis.  One face per fine cell on the interface (fine
+    tangential index first fastest); the coarse cell is fine index // r_d and
+    the subface offset o_d = (fine index % r_d) / r_d, size s_d = 1 / r_d.
+    The coarse side is interpolated at its mapped points o + s xi and its
+    weights carry s (the nonconforming face DofToQuad, facetquaddata.hpp:
+    86-155, used by GetWeight at advectionoperator.hpp:651), while the
+    geometry and field are evaluated at the coarse face's own, unmapped rule
+    points (ComputePhysicalCoordinatesAndJacobian with the mesh face data,
+    :646); pinned by GenDiL's fixtures, which differ by 10 % for a
+    non-constant field if X is mapped.  u_c / y_c, u_f / y_f: the two
+    spaces' parts of the vectors."""
+    D = len(coarse["sizes"])
+    axis, sign = face % D, (1 if face >= D else -1)
+    sc, sf = list(coarse["sizes"]), list(fine["sizes"])
+    r = [1 if d == axis else sf[d] // sc[d] for d in range(D)]
+    hc = np.asarray(coarse["h"], dtype=np.float64)
+    oc = np.asarray(coarse["origin"], dtype=np.float64)
+    xq, wq = points(1, q)
+    nodes_c, _ = points(coarse["basis"], coarse["n"])
+    nodes_f, _ = points(fine["basis"], fine["n"])
+    nc, nf = coarse["n"], fine["n"]
+    xi_m = 1.0 if sign > 0 else 0.0
+    bm = lagrange_matrix(nodes_c, [xi_m])[0]
+    bp = lagrange_matrix(nodes_f, [1.0 - xi_m])[0]
+    Bf = lagrange_matrix(nodes_f, xq)
+    tdims = [d for d in range(D) if d != axis]
+    grids = np.meshgrid(*([xq] * (D - 1)), indexing="ij")
+
+    def trace(u, n, b, Bs):
+        t = np.tensordot(u.reshape([n] * D, order="F"), b, axes=([axis], [0]))
+        for k in range(D - 1):
+            t = np.moveaxis(np.tensordot(Bs[k], t, axes=([1], [k])), 0, k)
+        return t
+
+    def spread(F, b, Bs):
+        t = F
+        for k in range(D - 1):
+            t = np.moveaxis(np.tensordot(Bs[k].T, t, axes=([1], [k])), 0, k)
+        return np.moveaxis(np.multiply.outer(t, b), D - 1, axis).reshape(-1, order="F")
+
+    fs = list(sf)
+    fs[axis] = 1
+    for f in range(int(np.prod(fs))):
+        ft = list(np.unravel_index(f, fs, order="F"))
+        im = [ft[d] // r[d] for d in range(D)]
+        im[axis] = sc[axis] - 1 if sign > 0 else 0
+        ip = list(ft)
+        ip[axis] = 0 if sign > 0 else sf[axis] - 1
+        em = int(np.ravel_multi_index(im, sc, order="F"))
+        ep = int(np.ravel_multi_index(ip, sf, order="F"))
+        Bm = [lagrange_matrix(nodes_c, (ft[d] % r[d] + xq) / r[d]) for d in tdims]
+        X = np.zeros([q] * (D - 1) + [D])
+        wts = np.ones([q] * (D - 1))
+        for k, d in enumerate(tdims):
+            X[..., d] = oc[d] + hc[d] * (im[d] + grids[k])
+            shape = [1] * (D - 1)
+            shape[k] = q
+            wts = wts * wq.reshape(shape) / r[d]
+        X[..., axis] = oc[axis] + hc[axis] * (im[axis] + xi_m)
+        dot = beta(X)[..., axis] * (sign / hc[axis])
+        um = trace(u_c[em * nc**D:(em + 1) * nc**D], nc, bm, Bm)
+        up = trace(u_f[ep * nf**D:(ep + 1) * nf**D], nf, bp, [Bf] * (D - 1))
+        F = wts * float(np.prod(hc)) * np.where(dot > 0.0, dot * um, dot * up)
+        y_c[em * nc**D:(em + 1) * nc**D] += spread(F, bm, Bm)
+        y_f[ep * nf**D:(ep + 1) * nf**D] -= spread(F, bp, [Bf] * (D - 1))

@@ -17,6 +17,8 @@
 //   op    : mass | adv | advface (face-centric operator on the interior faces)
 //           | intermesh_<F>_<PP> (3D, GLL: face F of an n^3 minus mesh of order
 //           P glued to an n^3 plus mesh of order PP, q shared; vector [minus|plus])
+//           | hmesh_<F>_<R> (GLL order P: face F of a coarse n^D mesh glued to
+//           a mesh R times finer along the face; vector [coarse|fine])
 //   basis : L (Gauss-Lobatto nodes) | G (Gauss-Legendre nodes)
 //   coef  : mass -> one | poly ;  adv -> c2 | bubble | const | trig | phase | yzx
 //   bc    : none (boundary faces skipped) | zero | poly
@@ -326,6 +328,56 @@ int run_intermesh( const Args & a )
 
 struct IntermeshEntry { int PM, PP, Q, F; Runner fn; };
 
+// GenDiL's nonconforming face operator over
+// NonconformingCartesianIntermeshFaceConnectivity<D, F>
+// (nonconformingcartesianintermeshfaceconnectivity.hpp:73-243): face F of a
+// coarse n^D mesh (minus, h = 1/n, at the origin) glued to a fine mesh (plus)
+// with R times the cells along every tangential axis (h = 1/(R n) there),
+// shifted by one unit along the face normal, GLL order P on both sides, Q
+// points shared, vector [coarse | fine] (as
+// tests/integration/gendil/test-advection-h-adaptivity-2d.cpp:66-140).
+template < int P, size_t... Is >
+auto make_gll( std::index_sequence< Is... > )
+{
+   return MakeLobattoFiniteElement( FiniteElementOrders< ( (void)Is, P )... >{} );
+}
+
+template < int D, int P, int Q, int F, int R >
+int run_hmesh( const Args & a )
+{
+   constexpr int axis = F % D;
+   std::array< GlobalIndex, D > sc, sf;
+   std::array< Real, D > hc, hf;
+   Point< D > oc, of;
+   for ( int d = 0; d < D; ++d )
+   {
+      sc[d] = a.n;
+      sf[d] = d == axis ? a.n : R * a.n;
+      hc[d] = 1.0 / a.n;
+      hf[d] = d == axis ? 1.0 / a.n : 1.0 / ( R * a.n );
+      oc[d] = 0.0;
+      of[d] = d == axis ? ( F >= D ? 1.0 : -1.0 ) : 0.0;
+   }
+   CartesianMesh< D > mesh_c( sc, hc, oc, false );
+   CartesianMesh< D > mesh_f( sf, hf, of, false );
+   auto fe = make_gll< P >( std::make_index_sequence< D >{} );
+   L2Restriction res_c{ 0 };
+   auto fes_c = MakeFiniteElementSpace( mesh_c, fe, res_c );
+   const Integer nc = fes_c.GetNumberOfFiniteElementDofs();
+   L2Restriction res_f{ nc };
+   auto fes_f = MakeFiniteElementSpace( mesh_f, fe, res_f );
+   const Integer nf = fes_f.GetNumberOfFiniteElementDofs();
+   auto ir = make_rule< Q >( std::make_index_sequence< D >{} );
+   NonconformingCartesianIntermeshFaceConnectivity< D, F > conn( sc, sf );
+   const std::string coef = a.coef;
+   auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+   auto op = MakeAdvectionFaceOperator< SerialKernelConfiguration >( fes_c, fes_f, conn, ir, adv );
+   return finish( a, op, nc + nf, fes_c.GetNumberOfFiniteElements() + fes_f.GetNumberOfFiniteElements() );
+}
+
+struct HmeshEntry { int D, P, Q, F, R; Runner fn; };
+
+
 // The operator of GenDiL's 6D benchmark (benchmarks/benchmark-advection-6d.cpp:
 // 14-98): a product of a 3D (h = 1), a 2D (h = 1.2345) and a 1D (h = 1)
 // Cartesian mesh with n elements per axis, Legendre p = 1 elements, 3 Gauss
@@ -364,6 +416,11 @@ const CaseEntry kEntries[] = {
    GD_CASE( 6, 1, 2 ), GD_CASE( 6, 2, 3 ),
 };
 
+const HmeshEntry kHmesh[] = {
+   { 2, 2, 4, 2, 2, &run_hmesh< 2, 2, 4, 2, 2 > }, { 3, 2, 3, 3, 2, &run_hmesh< 3, 2, 3, 3, 2 > },
+   { 3, 1, 2, 0, 2, &run_hmesh< 3, 1, 2, 0, 2 > }, { 3, 3, 4, 4, 3, &run_hmesh< 3, 3, 4, 4, 3 > },
+};
+
 const IntermeshEntry kIntermesh[] = {
    { 1, 3, 5, 3, &run_intermesh< 1, 3, 5, 3 > }, { 3, 1, 5, 0, &run_intermesh< 3, 1, 5, 0 > },
    { 2, 2, 4, 0, &run_intermesh< 2, 2, 4, 0 > }, { 2, 3, 4, 4, &run_intermesh< 2, 3, 4, 4 > },
@@ -394,6 +451,15 @@ int main( int argc, char ** argv )
    if ( a.mode == "fixture" || a.mode == "bicgstab" ) a.out = argv[12];
    else a.pairs = std::atoi( argv[12] );
    if ( a.op == "adv6dprod" ) return run_product( a );
+   if ( a.op.rfind( "hmesh_", 0 ) == 0 )
+   {
+      int F = -1, R = -1;
+      if ( std::sscanf( a.op.c_str(), "hmesh_%d_%d", &F, &R ) != 2 ) return 2;
+      for ( const HmeshEntry & e : kHmesh )
+         if ( e.D == a.D && e.P == a.P && e.Q == a.Q && e.F == F && e.R == R ) return e.fn( a );
+      std::fprintf( stderr, "hmesh case D=%d F=%d P=%d Q=%d R=%d not instantiated\n", a.D, F, a.P, a.Q, R );
+      return 2;
+   }
    if ( a.op.rfind( "intermesh_", 0 ) == 0 )
    {
       int F = -1, PP = -1;

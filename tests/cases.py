@@ -157,7 +157,7 @@ class FaceCase:
     along the face normal, q shared, vector [minus | plus]."""
 
     name: str
-    kind: str  # interior | intermesh
+    kind: str  # interior | intermesh | hmesh
     D: int
     P: int
     Q: int
@@ -167,9 +167,12 @@ class FaceCase:
     PP: int = 0
     face: int = 0
     seed: int = 12345
+    R: int = 1  # hmesh: fine cells per coarse cell along the tangential axes
 
     @property
     def harness_op(self):
+        if self.kind == "hmesh":
+            return f"hmesh_{self.face}_{self.R}"
         return "advface" if self.kind == "interior" else f"intermesh_{self.face}_{self.PP}"
 
     def sides(self):
@@ -179,6 +182,12 @@ class FaceCase:
         m = dict(sizes=[self.n] * self.D, h=[h] * self.D, origin=[0.0] * self.D, n=self.P + 1, basis=b)
         if self.kind == "interior":
             return m, m
+        if self.kind == "hmesh":
+            ax = self.face % self.D
+            fo = [0.0] * self.D
+            fo[ax] = 1.0 if self.face >= self.D else -1.0
+            return m, dict(sizes=[self.n if d == ax else self.R * self.n for d in range(self.D)],
+                           h=[h if d == ax else h / self.R for d in range(self.D)], origin=fo, n=self.P + 1, basis=b)
         po = [0.0] * self.D
         po[self.face % self.D] = 1.0 if self.face >= self.D else -1.0
         return m, dict(sizes=[self.n] * self.D, h=[h] * self.D, origin=po, n=self.PP + 1, basis=b)
@@ -193,7 +202,8 @@ class FaceCase:
             oracle.face_operator(m["sizes"], m["h"], m["origin"], m["n"], m["basis"], self.Q, beta, x, y)
         else:
             nm = self.n ** self.D * m["n"] ** self.D
-            oracle.intermesh_operator(self.face, self.Q, m, p, beta, x[:nm], x[nm:], y[:nm], y[nm:])
+            fn = oracle.hmesh_operator if self.kind == "hmesh" else oracle.intermesh_operator
+            fn(self.face, self.Q, m, p, beta, x[:nm], x[nm:], y[:nm], y[nm:])
         return y
 
     def gpu_operator(self):
@@ -205,6 +215,13 @@ class FaceCase:
             fes = G.MakeFiniteElementSpace(G.CartesianMesh.unit(self.D, self.n, False), fe(self.P))
             return G.MakeAdvectionFaceOperator(fes, G.MakeCartesianInteriorFaceConnectivity(fes.mesh.sizes), ir, beta)
         m, p = self.sides()
+        if self.kind == "hmesh":
+            mesh_c = G.CartesianMesh(m["sizes"], m["h"], m["origin"], False)
+            mesh_f = G.CartesianMesh(p["sizes"], p["h"], p["origin"], False)
+            fc = G.MakeFiniteElementSpace(mesh_c, fe(self.P))
+            ff = G.MakeFiniteElementSpace(mesh_f, fe(self.P), G.L2Restriction(fc.num_dofs))
+            conn = G.NonconformingCartesianIntermeshFaceConnectivity(self.D, self.face, mesh_c.sizes, mesh_f.sizes)
+            return G.MakeAdvectionFaceOperator(fc, ff, conn, ir, beta)
         mesh_m = G.Cartesian3DMesh(1.0 / self.n, self.n, self.n, self.n, origin=tuple(m["origin"]))
         mesh_p = G.Cartesian3DMesh(1.0 / self.n, self.n, self.n, self.n, origin=tuple(p["origin"]))
         fm = G.MakeFiniteElementSpace(mesh_m, fe(self.P))
@@ -225,6 +242,12 @@ FACE_CASES = [
     FaceCase("imesh3d_const_f0_p2p2q4", "intermesh", 3, 2, 4, "L", 3, "const", PP=2, face=0),
     FaceCase("imesh3d_c2_f4_p2p3q4", "intermesh", 3, 2, 4, "L", 3, "c2", PP=3, face=4),
     FaceCase("imesh3d_c2_f2_p3p2q4", "intermesh", 3, 3, 4, "L", 3, "c2", PP=2, face=2),
+    # h-nonconforming interfaces: test-advection-h-adaptivity-2d.cpp (2D, +x face,
+    # p = 2, q = 4, twice the cells along y on the fine side), then 3D ones
+    FaceCase("hmesh2d_const_f2_p2q4_r2", "hmesh", 2, 2, 4, "L", 2, "const", face=2, R=2),
+    FaceCase("hmesh3d_c2_f3_p2q3_r2", "hmesh", 3, 2, 3, "L", 2, "c2", face=3, R=2),
+    FaceCase("hmesh3d_c2_f0_p1q2_r2", "hmesh", 3, 1, 2, "L", 2, "c2", face=0, R=2),
+    FaceCase("hmesh3d_trig_f4_p3q4_r3", "hmesh", 3, 3, 4, "L", 2, "trig", face=4, R=3),
 ]
 
 FACE_BY_NAME = {c.name: c for c in FACE_CASES}

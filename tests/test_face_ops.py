@@ -232,3 +232,47 @@ def test_face_gpu_kernel_paths_against_oracle(D, p, q, n, coef):
     ref = np.zeros_like(x)
     oracle.face_operator([n] * D, [1.0 / n] * D, [0.0] * D, p + 1, 0, q, beta_numpy(coef, D), x, ref)
     assert rel_l2(y.cpu().numpy() - y0, ref) <= F64_TOL
+
+
+@pytest.mark.gpu
+def test_hmesh_gpu_ratio_one_is_the_conforming_interface_and_conserves():
+    # an h-interface with one fine cell per coarse cell is the conforming
+    # interface; with two per axis the fluxes still cancel (sum y = 0)
+    import torch
+    from cases import beta_field
+
+    n, p = 12, 3
+    h = 1.0 / n
+    fe = G.MakeLobattoFiniteElement(p)
+    ir = G.MakeIntegrationRule(p + 1)
+    beta = beta_field("c2", 3)
+    mc = G.Cartesian3DMesh(h, n, n, n)
+    mf = G.Cartesian3DMesh(h, n, n, n, origin=(1.0, 0.0, 0.0))
+    fc = G.MakeFiniteElementSpace(mc, fe)
+    ff = G.MakeFiniteElementSpace(mf, fe, G.L2Restriction(fc.num_dofs))
+    op_h = G.MakeAdvectionFaceOperator(fc, ff, G.NonconformingCartesianIntermeshFaceConnectivity(3, 3, mc.sizes, mf.sizes),
+                                       ir, beta)
+    op_c = G.MakeAdvectionFaceOperator(fc, ff, G.CartesianIntermeshFaceConnectivity(3, 3, mc.sizes, mf.sizes), ir, beta)
+    N = fc.num_dofs + ff.num_dofs
+    x = _dev(G.fill_random(N, 31))
+    y_h = torch.zeros(N, dtype=torch.float64, device="cuda")
+    y_c = torch.zeros(N, dtype=torch.float64, device="cuda")
+    op_h(x, y_h)
+    op_c(x, y_c)
+    assert rel_l2(y_h.cpu().numpy(), y_c.cpu().numpy()) <= 1e-14
+    mf2 = G.CartesianMesh([n, 2 * n, 2 * n], [h, h / 2, h / 2], [1.0, 0.0, 0.0], False)
+    ff2 = G.MakeFiniteElementSpace(mf2, fe, G.L2Restriction(fc.num_dofs))
+    op_2 = G.MakeAdvectionFaceOperator(fc, ff2, G.NonconformingCartesianIntermeshFaceConnectivity(3, 3, mc.sizes,
+                                                                                                   mf2.sizes), ir, beta)
+    N2 = fc.num_dofs + ff2.num_dofs
+    x2 = G.fill_random(N2, 32)
+    y2 = torch.zeros(N2, dtype=torch.float64, device="cuda")
+    op_2(_dev(x2), y2)
+    got = y2.cpu().numpy()
+    assert abs(got.sum()) <= 1e-13 * np.abs(got).sum()
+    ref = np.zeros(N2)
+    m, f = (dict(sizes=[n] * 3, h=[h] * 3, origin=[0.0] * 3, n=p + 1, basis=0),
+            dict(sizes=[n, 2 * n, 2 * n], h=[h, h / 2, h / 2], origin=[1.0, 0.0, 0.0], n=p + 1, basis=0))
+    nc = fc.num_dofs
+    oracle.hmesh_operator(3, p + 1, m, f, beta_numpy("c2", 3), x2[:nc], x2[nc:], ref[:nc], ref[nc:])
+    assert rel_l2(got, ref) <= F64_TOL
