@@ -33,6 +33,8 @@ constexpr int kThreads = 256;
 constexpr int kMaxS = 1024;             // largest face tensor, entries
 constexpr int kLdsBytes = 40 * 1024;    // dynamic LDS per block
 constexpr int kMaxFpb = 64;             // faces per block
+constexpr int kMaxQN = 1024;            // basis table staged in LDS by the one-launch kernel
+constexpr int kMaxN = 32;
 
 struct Side {
   const double* x;
@@ -290,11 +292,18 @@ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
 template <int DD, int NN, int QQ>
 __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
 {
+  // static LDS sized for the instance (fixed sizes: exactly what they need)
+  constexpr int kNT = DD > 0 ? cpow(NN, DD - 1) : kMaxS;
+  constexpr int kQN = DD > 0 ? NN * QQ : kMaxQN;
+  constexpr int kNB = DD > 0 ? NN : kMaxN;
   extern __shared__ double lds[];
-  __shared__ Idx ix;
+  __shared__ int64_t elem[2][kMaxFpb];    // the current face's minus and plus cells
+  __shared__ int offs[kNT];               // element-local offset of each tangential node
   __shared__ int sub[kMaxFpb][kMaxDim];   // owner cell indices
   __shared__ int subm[kMaxFpb][kMaxDim];  // the current face's minus cell
   __shared__ int ok[kMaxFpb];             // the face is interior
+  __shared__ double sB[kQN];              // basis at the rule points, [q][n]
+  __shared__ double sbf[2][kNB];          // basis at xi = 0 and xi = 1
   const Side& sd = a.s[0];
   const int64_t E = a.nfaces;  // elements
   const int64_t e0 = (int64_t)blockIdx.x * a.fpb;
@@ -318,24 +327,29 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
     }
   }
   for (int t = threadIdx.x; t < nf * nd; t += kThreads) acc[t] = 0.0;
+  for (int t = threadIdx.x; t < q * n; t += kThreads) sB[t] = sd.B[t];
+  for (int t = threadIdx.x; t < n; t += kThreads) {
+    sbf[0][t] = a.s[0].bf[t];
+    sbf[1][t] = a.s[1].bf[t];
+  }
   const Params<double>& p = a.geo;
   int64_t str = 1;
   for (int axis = 0; axis < D; ++axis) {
     const double scale = -1.0 / (double)p.h[axis];  // minus normal -e_axis
     for (int slot = 0; slot < 2; ++slot) {
       __syncthreads();  // the previous face's tables are no longer read
-      for (int t = threadIdx.x; t < nt; t += kThreads) ix.off[0][t] = dof_index(D, axis, n, t, 0);
+      for (int t = threadIdx.x; t < nt; t += kThreads) offs[t] = dof_index(D, axis, n, t, 0);
       for (int t = threadIdx.x; t < nf; t += kThreads) {
         const int64_t e = e0 + t;
         const bool in = slot == 0 ? sub[t][axis] > 0 : sub[t][axis] < sd.M[axis] - 1;
         ok[t] = in;
-        ix.elem[0][t] = in && slot == 1 ? e + str : e;  // minus: the upper cell
-        ix.elem[1][t] = in && slot == 0 ? e - str : e;  // plus: the lower cell
+        elem[0][t] = in && slot == 1 ? e + str : e;  // minus: the upper cell
+        elem[1][t] = in && slot == 0 ? e - str : e;  // plus: the lower cell
         for (int d = 0; d < D; ++d) subm[t][d] = sub[t][d] + ((in && slot == 1 && d == axis) ? 1 : 0);
       }
       __syncthreads();
-      interp_n(D, n, q, S, a.s[0].B, a.s[0].bf, a.s[0].x, axis, ix.elem[0], ix.off[0], nf, A, Bb, Um);
-      interp_n(D, n, q, S, a.s[1].B, a.s[1].bf, a.s[1].x, axis, ix.elem[1], ix.off[0], nf, A, Bb, Up);
+      interp_n(D, n, q, S, sB, sbf[0], sd.x, axis, elem[0], offs, nf, A, Bb, Um);
+      interp_n(D, n, q, S, sB, sbf[1], sd.x, axis, elem[1], offs, nf, A, Bb, Up);
       for (int t = threadIdx.x; t < nf * nq; t += kThreads) {
         const int f = t / nq;
         const int at = f * S + (t - f * nq);
@@ -361,22 +375,22 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
       }
       __syncthreads();
       // the owner's share: + b0 (x) B^T F on its lower face, - b1 (x) B^T F on its upper face
-      const Side& own = a.s[slot];
+      const double* ownbf = sbf[slot];
       const double sgn = slot == 0 ? 1.0 : -1.0;
       const int ax = ipow(n, axis);
       const double* src = Um;
       for (int kk = 0; kk < T; ++kk) {
         double* out = (kk % 2 == 0) ? A : Bb;
-        sweep<true>(own.B, n, q, n, ipow(n, kk), ipow(q, T - 1 - kk), nf, S, src, out);
+        sweep<true>(sB, n, q, n, ipow(n, kk), ipow(q, T - 1 - kk), nf, S, src, out);
         __syncthreads();
         src = out;
       }
       for (int t = threadIdx.x; t < nf * nd; t += kThreads) {
         const int f = t / nd, r = t - f * nd;
         const int i = r / nt, tt = r - i * nt;
-        const double w = own.bf[i];
+        const double w = ownbf[i];
         if (w == 0.0 || !ok[f]) continue;
-        acc[f * nd + ix.off[0][tt] + i * ax] += sgn * (w * src[f * S + tt]);
+        acc[f * nd + offs[tt] + i * ax] += sgn * (w * src[f * S + tt]);
       }
     }
     str *= sd.M[axis];
@@ -485,7 +499,7 @@ extern "C" int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, doubl
     for (int d = 0; d < D; ++d) nd *= n;
     const long long per = (4LL * a.S + nd) * (long long)sizeof(double);
     const int fpb = (int)std::min<long long>(kMaxFpb, kLdsBytes / per);
-    if (fpb >= 8) {
+    if (fpb >= 8 && n <= kMaxN && n * a.q <= kMaxQN) {
       // one launch: every element adds its own share of each of its faces
       a.fpb = fpb;
       a.nfaces = plan->E;
