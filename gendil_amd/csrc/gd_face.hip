@@ -32,6 +32,7 @@ namespace face {
 constexpr int kThreads = 256;
 constexpr int kMaxS = 1024;             // largest face tensor, entries
 constexpr int kLdsBytes = 40 * 1024;    // dynamic LDS per block
+constexpr int kOwnerLdsBytes = 24 * 1024;  // one-launch kernel: more, smaller blocks per CU
 constexpr int kMaxFpb = 64;             // faces per block
 constexpr int kMaxQN = 1024;            // basis table staged in LDS by the one-launch kernel
 constexpr int kMaxN = 32;
@@ -498,7 +499,7 @@ extern "C" int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, doubl
     long long nd = 1;
     for (int d = 0; d < D; ++d) nd *= n;
     const long long per = (4LL * a.S + nd) * (long long)sizeof(double);
-    const int fpb = (int)std::min<long long>(kMaxFpb, kLdsBytes / per);
+    const int fpb = (int)std::min<long long>(kMaxFpb, kOwnerLdsBytes / per);
     if (fpb >= 8 && n <= kMaxN && n * a.q <= kMaxQN) {
       // one launch: every element adds its own share of each of its faces
       a.fpb = fpb;
