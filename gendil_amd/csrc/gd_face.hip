@@ -32,7 +32,7 @@ namespace face {
 constexpr int kThreads = 256;
 constexpr int kMaxS = 1024;             // largest face tensor, entries
 constexpr int kLdsBytes = 40 * 1024;    // dynamic LDS per block
-constexpr int kOwnerLdsBytes = 24 * 1024;  // one-launch kernel: more, smaller blocks per CU
+constexpr int kOwnerLdsBytes = 16 * 1024;  // one-launch kernel: more, smaller blocks per CU
 constexpr int kMaxFpb = 64;             // faces per block
 constexpr int kMaxQN = 1024;            // basis table staged in LDS by the one-launch kernel
 constexpr int kMaxN = 32;
