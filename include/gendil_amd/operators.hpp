@@ -363,6 +363,127 @@ Operator MakeMassAdvectionOperator(const FiniteElementSpace<Dim>& fes, const Int
   return Operator(std::move(plan), mass_coef, adv_coef);
 }
 
+// ---- face-centric operators (accumulate into y) --------------------------------
+
+// The interior faces of a Cartesian mesh, every axis, no periodic wrap
+// (MakeCartesianInteriorFaceConnectivity, cartesianglobalfaceconnectivity.hpp:89-108).
+template <int Dim>
+struct CartesianInteriorFaceConnectivity {
+  std::array<int64_t, Dim> sizes{};
+};
+template <int Dim>
+CartesianInteriorFaceConnectivity<Dim> MakeCartesianInteriorFaceConnectivity(const std::array<int64_t, Dim>& sizes)
+{
+  return {sizes};
+}
+// Face `Face` of the minus mesh glued to the opposite face of the plus mesh
+// (cartesianintermeshfaceconnectivity.hpp:36-180); equal tangential sizes.
+template <int Dim, int Face>
+struct CartesianIntermeshFaceConnectivity {
+  std::array<int64_t, Dim> sizes_minus{}, sizes_plus{};
+};
+// Coarse (minus) against a mesh with an integer number of fine cells per
+// coarse cell along each tangential axis (plus)
+// (nonconformingcartesianintermeshfaceconnectivity.hpp:73-243).
+template <int Dim, int Face>
+struct NonconformingCartesianIntermeshFaceConnectivity {
+  std::array<int64_t, Dim> sizes_minus{}, sizes_plus{};
+};
+
+// y += the upwind fluxes of the face mesh, each face once: + into the minus
+// cell, - into the plus cell (AdvectionLocalFaceOperator,
+// advectionoperator.hpp:478-548, 596-676).  Two-space operators read and
+// write the two spaces' parts of one vector, at offsets shift_minus and
+// shift_plus (GenDiL's L2Restriction{shift}).
+class FaceOperator {
+ public:
+  enum Kind { kInterior, kIntermesh, kRefined };
+  FaceOperator(std::shared_ptr<Plan> minus, std::shared_ptr<Plan> plus, Kind kind, int face, int64_t shift_minus,
+               int64_t shift_plus)
+      : minus_(std::move(minus)), plus_(std::move(plus)), kind_(kind), face_(face), sm_(shift_minus), sp_(shift_plus)
+  {
+  }
+  void operator()(const Vector<double>& x, Vector<double>& y) const
+  {
+    const double* xd = x.ReadDeviceData();
+    double* yd = y.ReadWriteDeviceData();
+    if (kind_ == kInterior) {
+      if (x.Size() < sm_ + minus_->num_dofs() || y.Size() < sm_ + minus_->num_dofs())
+        throw Error(GD_ERR_INVALID, "FaceOperator: vector too short");
+      check(gd_adv_face_apply_f64(minus_->get(), xd + sm_, yd + sm_, stream_), "gd_adv_face_apply_f64");
+      return;
+    }
+    const int64_t need = std::max(sm_ + minus_->num_dofs(), sp_ + plus_->num_dofs());
+    if (x.Size() < need || y.Size() < need) throw Error(GD_ERR_INVALID, "FaceOperator: vector too short");
+    if (kind_ == kIntermesh)
+      check(gd_adv_intermesh_apply_f64(minus_->get(), plus_->get(), face_, xd + sm_, xd + sp_, yd + sm_, yd + sp_,
+                                       stream_),
+            "gd_adv_intermesh_apply_f64");
+    else
+      check(gd_adv_hmesh_apply_f64(minus_->get(), plus_->get(), face_, xd + sm_, xd + sp_, yd + sm_, yd + sp_,
+                                   stream_),
+            "gd_adv_hmesh_apply_f64");
+  }
+  void SetStream(gd_stream s) { stream_ = s; }
+
+ private:
+  std::shared_ptr<Plan> minus_, plus_;
+  Kind kind_;
+  int face_;
+  int64_t sm_, sp_;
+  gd_stream stream_ = nullptr;
+};
+
+// MakeAdvectionFaceOperator(fes, face_mesh, ir, adv) (advectionoperator.hpp:1546-1566)
+template <int Dim, typename Beta>
+FaceOperator MakeAdvectionFaceOperator(const FiniteElementSpace<Dim>& fes,
+                                       const CartesianInteriorFaceConnectivity<Dim>& conn, const IntegrationRule& ir,
+                                       const Beta& beta)
+{
+  if (conn.sizes != fes.mesh.sizes) throw Error(GD_ERR_INVALID, "face connectivity sizes differ from the mesh");
+  auto plan = std::make_shared<Plan>(fes, ir);
+  plan->set_beta(beta);
+  return FaceOperator(plan, plan, FaceOperator::kInterior, 0, 0, 0);
+}
+
+namespace detail {
+template <int Dim, typename Beta>
+FaceOperator two_space_face_operator(const FiniteElementSpace<Dim>& fm, const FiniteElementSpace<Dim>& fp,
+                                     const std::array<int64_t, Dim>& sm, const std::array<int64_t, Dim>& sp,
+                                     FaceOperator::Kind kind, int face, const IntegrationRule& ir, const Beta& beta,
+                                     int64_t shift_minus, int64_t shift_plus)
+{
+  if (sm != fm.mesh.sizes || sp != fp.mesh.sizes)
+    throw Error(GD_ERR_INVALID, "face connectivity sizes differ from the meshes");
+  auto pm = std::make_shared<Plan>(fm, ir);
+  auto pp = std::make_shared<Plan>(fp, ir);
+  pm->set_beta(beta);
+  const int64_t sp_default = shift_plus < 0 ? pm->num_dofs() : shift_plus;
+  return FaceOperator(pm, pp, kind, face, shift_minus, sp_default);
+}
+}  // namespace detail
+
+// MakeAdvectionFaceOperator(fes_minus, fes_plus, face_mesh, ir, adv)
+// (advectionoperator.hpp:1586-1607); by default the vector is [minus | plus].
+template <int Dim, int Face, typename Beta>
+FaceOperator MakeAdvectionFaceOperator(const FiniteElementSpace<Dim>& fm, const FiniteElementSpace<Dim>& fp,
+                                       const CartesianIntermeshFaceConnectivity<Dim, Face>& conn,
+                                       const IntegrationRule& ir, const Beta& beta, int64_t shift_minus = 0,
+                                       int64_t shift_plus = -1)
+{
+  return detail::two_space_face_operator<Dim>(fm, fp, conn.sizes_minus, conn.sizes_plus, FaceOperator::kIntermesh, Face,
+                                         ir, beta, shift_minus, shift_plus);
+}
+template <int Dim, int Face, typename Beta>
+FaceOperator MakeAdvectionFaceOperator(const FiniteElementSpace<Dim>& fm, const FiniteElementSpace<Dim>& fp,
+                                       const NonconformingCartesianIntermeshFaceConnectivity<Dim, Face>& conn,
+                                       const IntegrationRule& ir, const Beta& beta, int64_t shift_minus = 0,
+                                       int64_t shift_plus = -1)
+{
+  return detail::two_space_face_operator<Dim>(fm, fp, conn.sizes_minus, conn.sizes_plus, FaceOperator::kRefined, Face,
+                                         ir, beta, shift_minus, shift_plus);
+}
+
 // ---- solvers on Operator -------------------------------------------------------
 
 // Restarted GMRES (GMRES_no_alloc semantics, gmres.hpp:74-288); x holds x0.

@@ -115,6 +115,62 @@ int main()
       const double e32 = rel_l2(std::vector<double>(yf.begin(), yf.end()), ref.apply(xh));
       expect(e32 <= 1e-5, "advection 6^3 p=3 FP32 vs oracle (rel l2)", e32);
     }
+    // ---- face operators: tests/integration/gendil/test-advection-p-adaptivity.cpp ----
+    // two 3^3 meshes side by side, GLL p = 1 | p = 3, 5 shared points,
+    // beta = (y, z, x): with u = 1, the fluxes cancel over both spaces
+    {
+      const double h = 1.0 / 3.0;
+      ga::CartesianMesh<3> mL{{3, 3, 3}, {h, h, h}, {0.0, 0.0, 0.0}, {false, false, false}};
+      ga::CartesianMesh<3> mR{{3, 3, 3}, {h, h, h}, {1.0, 0.0, 0.0}, {false, false, false}};
+      auto fL = ga::MakeFiniteElementSpace(mL, ga::MakeLobattoFiniteElement(1));
+      auto fR = ga::MakeFiniteElementSpace(mR, ga::MakeLobattoFiniteElement(3));
+      ga::AffineField<3> yzx{};
+      yzx.A[0][1] = yzx.A[1][2] = yzx.A[2][0] = 1.0;
+      ga::CartesianIntermeshFaceConnectivity<3, 3> iface{mL.sizes, mR.sizes};
+      auto op = ga::MakeAdvectionFaceOperator(fL, fR, iface, ga::MakeIntegrationRule(5), yzx);
+      const int64_t n = 27 * 8 + 27 * 64;
+      ga::Vector<double> u(n), r(n);
+      u.FromHost(std::vector<double>((size_t)n, 1.0));
+      r.FromHost(std::vector<double>((size_t)n, 0.0));
+      op(u, r);
+      const std::vector<double> rh = r.ToHost();
+      double sum = 0.0, abs_sum = 0.0;
+      for (double v : rh) {
+        sum += v;
+        abs_sum += std::fabs(v);
+      }
+      expect(std::fabs(sum) <= 1e-10 && abs_sum > 0.0, "p-adaptivity interface: |sum r| for u = 1", std::fabs(sum));
+      // interior faces of one mesh and the h-refined interface: conservation for random u
+      auto interior = ga::MakeAdvectionFaceOperator(fL, ga::MakeCartesianInteriorFaceConnectivity<3>(mL.sizes),
+                                                    ga::MakeIntegrationRule(2), yzx);
+      ga::Vector<double> x(27 * 8), y(27 * 8);
+      x.FromHost(ga::FillRandomHost(27 * 8, 5));
+      y.FromHost(std::vector<double>(27 * 8, 0.0));
+      interior(x, y);
+      double s2 = 0.0, a2 = 0.0;
+      for (double v : y.ToHost()) {
+        s2 += v;
+        a2 += std::fabs(v);
+      }
+      expect(std::fabs(s2) <= 1e-13 * a2, "interior faces 3^3 p=1: |sum y| / sum |y|", std::fabs(s2) / a2);
+      ga::CartesianMesh<3> mF{{3, 6, 6}, {h, h / 2, h / 2}, {1.0, 0.0, 0.0}, {false, false, false}};
+      auto fF = ga::MakeFiniteElementSpace(mF, ga::MakeLobattoFiniteElement(2));
+      auto fC = ga::MakeFiniteElementSpace(mL, ga::MakeLobattoFiniteElement(2));
+      ga::NonconformingCartesianIntermeshFaceConnectivity<3, 3> hface{mL.sizes, mF.sizes};
+      auto hop = ga::MakeAdvectionFaceOperator(fC, fF, hface, ga::MakeIntegrationRule(3), yzx);
+      const int64_t nh = 27 * 27 + 108 * 27;
+      ga::Vector<double> xh(nh), yh(nh);
+      xh.FromHost(ga::FillRandomHost(nh, 6));
+      yh.FromHost(std::vector<double>((size_t)nh, 0.0));
+      hop(xh, yh);
+      double s3 = 0.0, a3 = 0.0;
+      for (double v : yh.ToHost()) {
+        s3 += v;
+        a3 += std::fabs(v);
+      }
+      expect(std::fabs(s3) <= 1e-13 * a3 && a3 > 0.0, "h interface 3^3 | 3x6x6 p=2: |sum y| / sum |y|",
+             std::fabs(s3) / a3);
+    }
     // ---- CG on the mass matrix ----
     {
       auto mesh = ga::MakeUnitCartesianMesh<3>(5, false);
