@@ -164,6 +164,38 @@ __device__ __forceinline__ void interp_n(int D, int n, int q, int S, const doubl
   }
 }
 
+// Both sides of nf faces at once (one trace pass, one pass per sweep): faces
+// [0, nf) are the minus side (bf0, em), [nf, 2 nf) the plus side (bf1, ep).
+// The sweeps alternate between A and U so that the last lands in U.
+__device__ __forceinline__ void interp_pair(int D, int n, int q, int S, const double* B, const double* bf0,
+                                            const double* bf1, const double* x, int axis, const int64_t* em,
+                                            const int64_t* ep, const int* off, int nf, double* A, double* U)
+{
+  const int T = D - 1;
+  const int nt = ipow(n, T), nd = nt * n, ax = ipow(n, axis);
+  double* first = (T % 2 == 1) ? A : U;
+  for (int t = threadIdx.x; t < 2 * nf * nt; t += kThreads) {
+    const int f = t / nt, tt = t - f * nt;
+    const bool plus = f >= nf;
+    const double* bf = plus ? bf1 : bf0;
+    const double* u = x + (plus ? ep[f - nf] : em[f]) * nd + off[tt];
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double w = bf[i];
+      if (w != 0.0) s += w * u[i * ax];
+    }
+    first[f * S + tt] = s;
+  }
+  __syncthreads();
+  double* src = first;
+  for (int k = 0; k < T; ++k) {
+    double* out = src == A ? U : A;
+    sweep<false>(B, n, n, q, ipow(q, k), ipow(n, T - 1 - k), 2 * nf, S, src, out);
+    __syncthreads();
+    src = out;
+  }
+}
+
 __device__ void interp_side(const Args& a, const Side& sd, int axis, const int64_t* elem, const int* off, int nf,
                             double* A, double* Bb, double* dst, const int* foff = nullptr)
 {
@@ -315,11 +347,9 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
   const int S = DD > 0 ? cpow(NN > QQ ? NN : QQ, DD - 1) : a.S;
   const int T = D - 1;
   const int nt = ipow(n, T), nd = nt * n, nq = ipow(q, T);
-  double* A = lds;
-  double* Bb = A + a.fpb * S;
-  double* Um = Bb + a.fpb * S;
-  double* Up = Um + a.fpb * S;
-  double* acc = Up + a.fpb * S;
+  double* A = lds;                 // 2 fpb faces of scratch
+  double* U = A + 2 * a.fpb * S;   // minus faces, then plus faces
+  double* acc = U + 2 * a.fpb * S;
   for (int t = threadIdx.x; t < nf; t += kThreads) {
     int64_t r = e0 + t;
     for (int d = 0; d < D; ++d) {
@@ -349,13 +379,12 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
         for (int d = 0; d < D; ++d) subm[t][d] = sub[t][d] + ((in && slot == 1 && d == axis) ? 1 : 0);
       }
       __syncthreads();
-      interp_n(D, n, q, S, sB, sbf[0], sd.x, axis, elem[0], offs, nf, A, Bb, Um);
-      interp_n(D, n, q, S, sB, sbf[1], sd.x, axis, elem[1], offs, nf, A, Bb, Up);
+      interp_pair(D, n, q, S, sB, sbf[0], sbf[1], sd.x, axis, elem[0], elem[1], offs, nf, A, U);
       for (int t = threadIdx.x; t < nf * nq; t += kThreads) {
         const int f = t / nq;
         const int at = f * S + (t - f * nq);
         if (!ok[f]) {
-          Um[at] = 0.0;
+          U[at] = 0.0;
           continue;
         }
         int r = t - f * nq;
@@ -371,17 +400,17 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
           }
         }
         const double dot = beta_axis(p, axis, subm[f], k, q, D) * scale;
-        const double up = dot > 0.0 ? dot * Um[at] : dot * Up[at];
-        Um[at] = w * p.detJ * up;
+        const double up = dot > 0.0 ? dot * U[at] : dot * U[nf * S + at];
+        U[at] = w * p.detJ * up;
       }
       __syncthreads();
       // the owner's share: + b0 (x) B^T F on its lower face, - b1 (x) B^T F on its upper face
       const double* ownbf = sbf[slot];
       const double sgn = slot == 0 ? 1.0 : -1.0;
       const int ax = ipow(n, axis);
-      const double* src = Um;
+      const double* src = U;
       for (int kk = 0; kk < T; ++kk) {
-        double* out = (kk % 2 == 0) ? A : Bb;
+        double* out = src == A ? U : A;
         sweep<true>(sB, n, q, n, ipow(n, kk), ipow(q, T - 1 - kk), nf, S, src, out);
         __syncthreads();
         src = out;
