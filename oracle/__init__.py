@@ -13,7 +13,7 @@ Parity status: gdo_apply pinned by golden vectors from _ref/gendil_ref
 dot-product identity; gdo_grad_beta pinned by central finite differences;
 the p-adaptive apply pinned by its uniform special case and by conservation;
 bicgstab pinned by GenDiL's own BiCGSTAB_no_alloc run through _ref/gendil_ref
-(tests/golden/bicgstab3d_c2_p3q4_bc0.npz) and by dense solves; the face-centric,
+(tests/golden/bicgstab3d_c2_p1q2_bc0.npz) and by dense solves; the face-centric,
 intermesh and h-refined face operators (advection_faces, hmesh_operator) pinned
 by GenDiL's own face operators through _ref/gendil_ref (tests/golden/face*,
 imesh*, hmesh*), conservation and the split-mesh identity.
