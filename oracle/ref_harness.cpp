@@ -274,18 +274,6 @@ int run( const Args & a )
       auto op = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv, g );
       return finish( a, op, ndofs, nelem );
    }
-   if ( a.op == "advface" )
-   {
-      // AdvectionFaceOperator over CartesianInteriorFaceConnectivity
-      // (advectionoperator.hpp:1105-1180, cartesianglobalfaceconnectivity.hpp:15-95):
-      // every interior face once, y accumulated (+ minus side, - plus side)
-      auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
-      std::array< GlobalIndex, D > sizes;
-      for ( int d = 0; d < D; ++d ) sizes[d] = a.n;
-      auto conn = MakeCartesianInteriorFaceConnectivity< D >( sizes );
-      auto op = MakeAdvectionFaceOperator< SerialKernelConfiguration >( fes, conn, ir, adv );
-      return finish( a, op, ndofs, nelem );
-   }
    std::fprintf( stderr, "unknown op %s\n", a.op.c_str() );
    return 2;
 }
@@ -378,6 +366,23 @@ int run_hmesh( const Args & a )
 struct HmeshEntry { int D, P, Q, F, R; Runner fn; };
 
 
+// AdvectionFaceOperator over CartesianInteriorFaceConnectivity
+// (advectionoperator.hpp:1105-1180, cartesianglobalfaceconnectivity.hpp:15-95):
+// every interior face once, y accumulated (+ minus side, - plus side)
+template < int D, int P, int Q, bool Lobatto >
+int run_face( const Args & a )
+{
+   auto fes = make_space< D, P, Q, Lobatto >( a, std::make_index_sequence< D >{} );
+   auto ir = make_rule< Q >( std::make_index_sequence< D >{} );
+   const std::string coef = a.coef;
+   auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+   std::array< GlobalIndex, D > sizes;
+   for ( int d = 0; d < D; ++d ) sizes[d] = a.n;
+   auto conn = MakeCartesianInteriorFaceConnectivity< D >( sizes );
+   auto op = MakeAdvectionFaceOperator< SerialKernelConfiguration >( fes, conn, ir, adv );
+   return finish( a, op, fes.GetNumberOfFiniteElementDofs(), fes.GetNumberOfFiniteElements() );
+}
+
 // The operator of GenDiL's 6D benchmark (benchmarks/benchmark-advection-6d.cpp:
 // 14-98): a product of a 3D (h = 1), a 2D (h = 1.2345) and a 1D (h = 1)
 // Cartesian mesh with n elements per axis, Legendre p = 1 elements, 3 Gauss
@@ -416,6 +421,11 @@ const CaseEntry kEntries[] = {
    GD_CASE( 6, 1, 2 ), GD_CASE( 6, 2, 3 ),
 };
 
+const CaseEntry kFaceEntries[] = {
+   { 3, 3, 4, 'L', &run_face< 3, 3, 4, true > }, { 3, 2, 3, 'G', &run_face< 3, 2, 3, false > },
+   { 2, 2, 3, 'L', &run_face< 2, 2, 3, true > }, { 6, 1, 2, 'L', &run_face< 6, 1, 2, true > },
+};
+
 const HmeshEntry kHmesh[] = {
    { 2, 2, 4, 2, 2, &run_hmesh< 2, 2, 4, 2, 2 > }, { 3, 2, 3, 3, 2, &run_hmesh< 3, 2, 3, 3, 2 > },
    { 3, 1, 2, 0, 2, &run_hmesh< 3, 1, 2, 0, 2 > }, { 3, 3, 4, 4, 3, &run_hmesh< 3, 3, 4, 4, 3 > },
@@ -451,6 +461,13 @@ int main( int argc, char ** argv )
    if ( a.mode == "fixture" || a.mode == "bicgstab" ) a.out = argv[12];
    else a.pairs = std::atoi( argv[12] );
    if ( a.op == "adv6dprod" ) return run_product( a );
+   if ( a.op == "advface" )
+   {
+      for ( const CaseEntry & e : kFaceEntries )
+         if ( e.D == a.D && e.P == a.P && e.Q == a.Q && e.basis == a.basis[0] ) return e.fn( a );
+      std::fprintf( stderr, "advface case D=%d P=%d Q=%d basis=%s not instantiated\n", a.D, a.P, a.Q, a.basis.c_str() );
+      return 2;
+   }
    if ( a.op.rfind( "hmesh_", 0 ) == 0 )
    {
       int F = -1, R = -1;
