@@ -276,3 +276,28 @@ def test_hmesh_gpu_ratio_one_is_the_conforming_interface_and_conserves():
     nc = fc.num_dofs
     oracle.hmesh_operator(3, p + 1, m, f, beta_numpy("c2", 3), x2[:nc], x2[nc:], ref[:nc], ref[nc:])
     assert rel_l2(got, ref) <= F64_TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sizes", [(1, 4, 3), (1, 1, 1), (5, 1, 2)])
+def test_face_gpu_degenerate_meshes(sizes):
+    # axes with a single cell have no interior faces; a single cell has none at all
+    import torch
+    from cases import beta_field
+
+    h = [1.0 / s for s in sizes]
+    mesh = G.CartesianMesh(list(sizes), h, [0.0] * 3, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(2))
+    op = G.MakeAdvectionFaceOperator(fes, G.MakeCartesianInteriorFaceConnectivity(mesh.sizes), G.MakeIntegrationRule(3),
+                                     beta_field("c2", 3))
+    x = G.fill_random(fes.num_dofs, 41)
+    y0 = G.fill_random(fes.num_dofs, 42)
+    y = _dev(y0)
+    op(_dev(x), y)
+    ref = np.zeros_like(x)
+    oracle.face_operator(list(sizes), h, [0.0] * 3, 3, 0, 3, beta_numpy("c2", 3), x, ref)
+    got = y.cpu().numpy() - y0
+    if np.abs(ref).max() == 0.0:
+        assert np.abs(got).max() == 0.0
+    else:
+        assert rel_l2(got, ref) <= F64_TOL
