@@ -363,6 +363,15 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
     sbf[0][t] = a.s[0].bf[t];
     sbf[1][t] = a.s[1].bf[t];
   }
+  __shared__ int nzi[2][kNB];  // nodes with a nonzero basis value at xi = 0 / xi = 1
+  __shared__ int nzc[2];
+  if (threadIdx.x < 2) {
+    const double* bf = a.s[threadIdx.x].bf;
+    int c = 0;
+    for (int i = 0; i < n; ++i)
+      if (bf[i] != 0.0) nzi[threadIdx.x][c++] = i;
+    nzc[threadIdx.x] = c;
+  }
   const Params<double>& p = a.geo;
   int64_t str = 1;
   for (int axis = 0; axis < D; ++axis) {
@@ -415,12 +424,14 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
         __syncthreads();
         src = out;
       }
-      for (int t = threadIdx.x; t < nf * nd; t += kThreads) {
-        const int f = t / nd, r = t - f * nd;
-        const int i = r / nt, tt = r - i * nt;
-        const double w = ownbf[i];
-        if (w == 0.0 || !ok[f]) continue;
-        acc[f * nd + offs[tt] + i * ax] += sgn * (w * src[f * S + tt]);
+      // only the nodes with a nonzero basis value at the face (one for GLL)
+      const int nz = nzc[slot];
+      for (int t = threadIdx.x; t < nf * nz * nt; t += kThreads) {
+        const int f = t / (nz * nt), r = t - f * (nz * nt);
+        const int j = r / nt, tt = r - j * nt;
+        if (!ok[f]) continue;
+        const int i = nzi[slot][j];
+        acc[f * nd + offs[tt] + i * ax] += sgn * (ownbf[i] * src[f * S + tt]);
       }
     }
     str *= sd.M[axis];
