@@ -339,7 +339,12 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
   __shared__ double sbf[2][kNB];          // basis at xi = 0 and xi = 1
   const Side& sd = a.s[0];
   const int64_t E = a.nfaces;  // elements
-  const int64_t e0 = (int64_t)blockIdx.x * a.fpb;
+  // blocks go to the 8 XCDs round robin; give each XCD one contiguous run of
+  // element blocks so that a block's y and z neighbours were read into the
+  // same L2 (a bijection when the grid is a multiple of 8)
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int blk = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int64_t e0 = (int64_t)blk * a.fpb;
   const int nf = (int)min((int64_t)a.fpb, E - e0);
   const int D = DD > 0 ? DD : a.D;
   const int n = NN > 0 ? NN : sd.n;
