@@ -2,7 +2,6 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
-#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -53,8 +52,8 @@ int check_adv(const gd_plan* plan, bool need_pointwise)
   return GD_OK;
 }
 
-// 3D kernel choice: from the tuning table (autotuner), overridable with
-// GENDIL_AMD_K3_LANES.  0 = generic kernel; 1, 2, 4 = register-resident kernel
+// 3D kernel choice: from the tuning table (autotuner or gd_plan_set_variant).
+// 0 = generic kernel; 1, 2, 4 = register-resident kernel
 // with that many lanes per element; 16 + lanes = Kronecker-factored kernel
 // (lanes 1, 2, 4: register-resident; 32..39: LDS-staged, bit 0 = outputs
 // staged through LDS, bit 1 = 128-thread blocks, bit 2 = one term at a time).  Default: the staged
@@ -63,7 +62,6 @@ int check_adv(const gd_plan* plan, bool need_pointwise)
 template <typename T>
 int k3_choice(const gd_plan* plan, int N, bool adv, bool mass)
 {
-  if (const char* env = std::getenv("GENDIL_AMD_K3_LANES")) return std::atoi(env);
   const int tuned = tuned_lanes(plan, N, sizeof(T));
   if (tuned >= 0) return tuned;
   const int ks = k3k::default_staged(N, sizeof(T));
@@ -93,9 +91,7 @@ int padapt_apply(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y,
     const int rc = check_adv(plan, false);
     if (rc) return rc;
   }
-  int v = plan->tuned_pa[sizeof(T) == 8 ? 0 : 1];
-  if (const char* env = std::getenv("GENDIL_AMD_PA_VARIANT")) v = std::atoi(env);
-  return pa::apply<T>(plan, v, mass_coef, adv_coef, x, y, s);
+  return pa::apply<T>(plan, plan->tuned_pa[sizeof(T) == 8 ? 0 : 1], mass_coef, adv_coef, x, y, s);
 }
 
 template <typename T>

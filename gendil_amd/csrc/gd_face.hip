@@ -448,13 +448,29 @@ __global__ void __launch_bounds__(kThreads) k_faces_owner(Args a)
 
 using OwnerKernel = void (*)(Args);
 
+// faces per block of the one-launch kernel for (D, n, q): its LDS holds four
+// face tensors of q^(D-1) values and one element of n^D per face
+constexpr long long owner_fpb(int D, int n, int q)
+{
+  long long S = 1, nd = 1;
+  for (int d = 0; d < D; ++d) {
+    nd *= n;
+    if (d > 0) S *= q;
+  }
+  const long long f = kOwnerLdsBytes / ((4 * S + nd) * (long long)sizeof(double));
+  return f < kMaxFpb ? f : kMaxFpb;
+}
+
 OwnerKernel owner_kernel(int D, int n, int q)
 {
-#define GD_FACE_OWNER(d, nn, qq) \
+  // an instance is compiled only where the one-launch path can select it
+  // (at least 8 faces per block, as gd_adv_face_apply_f64 requires)
+#define GD_FACE_OWNER(d, nn, qq)                                                      \
+  static_assert(owner_fpb(d, nn, qq) >= 8, "face instance the one launch never selects"); \
   if (D == d && n == nn && q == qq) return k_faces_owner<d, nn, qq>;
   GD_FACE_OWNER(3, 2, 2) GD_FACE_OWNER(3, 2, 3) GD_FACE_OWNER(3, 3, 3) GD_FACE_OWNER(3, 3, 4)
-  GD_FACE_OWNER(3, 4, 4) GD_FACE_OWNER(3, 4, 5) GD_FACE_OWNER(3, 5, 6) GD_FACE_OWNER(2, 3, 3)
-  GD_FACE_OWNER(2, 4, 4) GD_FACE_OWNER(6, 2, 2) GD_FACE_OWNER(6, 2, 3)
+  GD_FACE_OWNER(3, 4, 4) GD_FACE_OWNER(3, 4, 5) GD_FACE_OWNER(2, 3, 3)
+  GD_FACE_OWNER(2, 4, 4) GD_FACE_OWNER(6, 2, 2)
 #undef GD_FACE_OWNER
   return k_faces_owner<0, 0, 0>;
 }
@@ -527,6 +543,22 @@ int common_args(const gd_plan* m, const gd_plan* p, int axis, int sign, Args& a,
   return GD_OK;
 }
 
+// true when [a, a + na) and [b, b + nb) share storage
+bool overlap(const double* a, int64_t na, const double* b, int64_t nb)
+{
+  const uintptr_t a0 = reinterpret_cast<uintptr_t>(a), b0 = reinterpret_cast<uintptr_t>(b);
+  return a0 < b0 + (uintptr_t)nb * sizeof(double) && b0 < a0 + (uintptr_t)na * sizeof(double);
+}
+
+// x is read while y is added into (in several launches for the coloured and
+// h-refined paths), so no input side may share storage with an output side
+bool sides_overlap(const gd_plan* m, const gd_plan* p, const double* xm, const double* xp, const double* ym,
+                   const double* yp)
+{
+  return overlap(xm, m->ndofs, ym, m->ndofs) || overlap(xm, m->ndofs, yp, p->ndofs) ||
+         overlap(xp, p->ndofs, ym, m->ndofs) || overlap(xp, p->ndofs, yp, p->ndofs);
+}
+
 }  // namespace face
 }  // namespace gd
 
@@ -535,6 +567,8 @@ extern "C" int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, doubl
   using namespace gd::face;
   if (int rc = check_plan(plan, "gd_adv_face_apply_f64")) return rc;
   if (plan->ndofs > 0 && (!x || !y)) return gd::fail(GD_ERR_INVALID, "gd_adv_face_apply_f64: null vector");
+  if (overlap(x, plan->ndofs, y, plan->ndofs))
+    return gd::fail(GD_ERR_INVALID, "gd_adv_face_apply_f64: x and y must not overlap");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int D = plan->dim;
   {
@@ -594,6 +628,8 @@ extern "C" int gd_adv_intermesh_apply_f64(const gd_plan* minus, const gd_plan* p
   if (face < 0 || face >= 2 * D) return gd::fail(GD_ERR_INVALID, "gd_adv_intermesh_apply_f64: face out of range");
   if (!x_minus || !x_plus || !y_minus || !y_plus)
     return gd::fail(GD_ERR_INVALID, "gd_adv_intermesh_apply_f64: null vector");
+  if (sides_overlap(minus, plus, x_minus, x_plus, y_minus, y_plus))
+    return gd::fail(GD_ERR_INVALID, "gd_adv_intermesh_apply_f64: x and y must not overlap");
   const int axis = face % D, sign = face >= D ? 1 : -1;
   Args a;
   if (int rc = common_args(minus, plus, axis, sign, a)) return rc;
@@ -677,6 +713,8 @@ extern "C" int gd_adv_hmesh_apply_f64(const gd_plan* coarse, const gd_plan* fine
   if (face < 0 || face >= 2 * D) return gd::fail(GD_ERR_INVALID, "gd_adv_hmesh_apply_f64: face out of range");
   if (!x_coarse || !x_fine || !y_coarse || !y_fine)
     return gd::fail(GD_ERR_INVALID, "gd_adv_hmesh_apply_f64: null vector");
+  if (sides_overlap(coarse, fine, x_coarse, x_fine, y_coarse, y_fine))
+    return gd::fail(GD_ERR_INVALID, "gd_adv_hmesh_apply_f64: x and y must not overlap");
   const int axis = face % D, sign = face >= D ? 1 : -1;
   int r[gd::kMaxDim];
   double wscale = 1.0;

@@ -33,10 +33,15 @@ KList<float> instances_f32_n2();
 KList<float> instances_f32_n3();
 KList<float> instances_f32_n4();
 
-// Plan-side cache: per-position factor tables (both precisions), rebuilt when
-// the plan's coefficients change.
+// Plan-side cache: per-position factor tables (both precisions).
 struct State;
 void destroy_state(State* s);
+
+// Rebuild the factor tables for the plan's current coefficients (allocates
+// and copies synchronously).  Called by plan creation and by every
+// coefficient setter, after the device has drained, so applies never
+// allocate or copy.
+int refresh(const gd_plan* plan);
 
 // y = mass_coef M x + adv_coef A x if an instance serves this plan and lane
 // count; GD_ERR_UNSUPPORTED (nothing launched) otherwise.

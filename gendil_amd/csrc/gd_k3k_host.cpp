@@ -91,15 +91,16 @@ int upload(const std::vector<double>& h, void** dst)
   return GD_OK;
 }
 
-// (Re)build the factor tables for the plan's current beta.
+}  // namespace
+
 int refresh(const gd_plan* plan)
 {
   if (!plan->k3k_state) plan->k3k_state = new State();
   State& st = *plan->k3k_state;
-  if (st.version == plan->coef_version) return GD_OK;
   st.version = plan->coef_version;
   st.ok = false;
   st.negmask = 0;
+  if (plan->dim != 3 || plan->padapt || plan->desc.basis != GD_BASIS_GLL) return GD_OK;
   if (plan->beta_kind != GD_BETA_SEPARABLE) return GD_OK;
   const int N = plan->desc.order + 1, Q = N;
   const TableSet* ts = find_tables(plan, N, Q);
@@ -161,6 +162,8 @@ int refresh(const gd_plan* plan)
   return GD_OK;
 }
 
+namespace {
+
 template <typename T>
 bool prepare(const gd_plan* plan, int lanes, bool adv, bool mass, KParams<T>& prm, KLaunch<T>* fn)
 {
@@ -177,9 +180,9 @@ bool prepare(const gd_plan* plan, int lanes, bool adv, bool mass, KParams<T>& pr
       double g = 0.0;
       if (!constant_of(plan->bc_host, &g) || g != 0.0) return false;
     }
-    if (refresh(plan) != GD_OK) return false;
+    // the factor tables are rebuilt by the coefficient setters (refresh)
     st = plan->k3k_state;
-    if (!st->ok) return false;
+    if (!st || st->version != plan->coef_version || !st->ok) return false;
     sigc = ksig(st->axis[0], st->axis[1], st->axis[2]);
   }
   const KList<T> l = list_for<T>(N);

@@ -1,4 +1,4 @@
-// Host-side interface of the p-adaptive persistent apply (gd_padapt.hpp).
+// Host-side interface of the p-adaptive apply (gd_padapt.hpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,6 +38,12 @@ int resolved_variant(const gd_plan* plan, int v);
 // Plan-side state (element buckets, device parameter copies).
 struct State;
 void destroy_state(State* s);
+
+// Build the element buckets (once) and upload the parameter blocks of both
+// precisions for the plan's current coefficients (synchronous copies).
+// Called by plan creation and by every coefficient setter, after the device
+// has drained, so applies never allocate or copy.
+int refresh(const gd_plan* plan);
 
 // Is the plan served by this path?  On false, *why says what is missing.
 bool eligible(const gd_plan* plan, const char** why);

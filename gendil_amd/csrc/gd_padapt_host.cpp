@@ -188,6 +188,21 @@ bool eligible(const gd_plan* plan, const char** why)
   return true;
 }
 
+int refresh(const gd_plan* plan)
+{
+  if (!plan->padapt) return GD_OK;
+  if (int rc = ensure_state(plan)) return rc;
+  if (!eligible(plan, nullptr)) return GD_OK;
+  State& st = *plan->pa_state;
+  const PParams<double> h64 = host_params<double>(plan, st);
+  const PParams<float> h32 = host_params<float>(plan, st);
+  GD_HIP(hipMemcpy(st.params[0], &h64, sizeof(h64), hipMemcpyHostToDevice));
+  GD_HIP(hipMemcpy(st.params[1], &h32, sizeof(h32), hipMemcpyHostToDevice));
+  st.version[0] = st.version[1] = plan->coef_version;
+  st.kron_ok = h64.kron_ok != 0;
+  return GD_OK;
+}
+
 template <typename T>
 int apply(const gd_plan* plan, int v, T mass_coef, T adv_coef, const T* x, T* y, hipStream_t s)
 {
@@ -195,12 +210,9 @@ int apply(const gd_plan* plan, int v, T mass_coef, T adv_coef, const T* x, T* y,
   if (!eligible(plan, &why)) return fail(GD_ERR_UNSUPPORTED, why);
   State& st = *plan->pa_state;
   const int slot = sizeof(T) == 8 ? 0 : 1;
-  if (st.version[slot] != plan->coef_version) {
-    const PParams<T> hp = host_params<T>(plan, st);
-    GD_HIP(hipMemcpy(st.params[slot], &hp, sizeof(hp), hipMemcpyHostToDevice));
-    st.version[slot] = plan->coef_version;
-    st.kron_ok = hp.kron_ok != 0;
-  }
+  // the parameter block is rebuilt by the coefficient setters (refresh)
+  if (st.version[slot] != plan->coef_version)
+    return fail(GD_ERR_INVALID, "p-adaptive apply: device parameters out of date");
   const bool kron = plan->dim == 6 && st.kron_ok;
   if (v < 0) v = kron ? 2 : 0;
   if (v >= num_variants(plan->dim) || (v == 2 && !kron)) v = 0;

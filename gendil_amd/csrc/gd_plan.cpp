@@ -12,6 +12,8 @@
 #include <utility>
 
 #include "gd_internal.hpp"
+#include "gd_k3k_api.hpp"
+#include "gd_padapt_api.hpp"
 
 namespace gd {
 
@@ -278,6 +280,22 @@ int upload_fields(gd_plan* p)
   return GD_OK;
 }
 
+// Every coefficient change ends here.  Applies queued earlier on any stream
+// read the plan's device tables, so wait for the device to drain before they
+// are replaced; then rebuild what the kernels read (separable field tables,
+// the Kronecker kernel's factor tables, the p-adaptive parameter blocks), so
+// the apply entry points never allocate or copy.
+int commit_coefficients(gd_plan* p, bool fields)
+{
+  ++p->coef_version;
+  GD_HIP(hipDeviceSynchronize());
+  if (fields) {
+    if (int rc = upload_fields(p)) return rc;
+  }
+  if (int rc = k3k::refresh(p)) return rc;
+  return pa::refresh(p);
+}
+
 }  // namespace
 
 const TableSet* find_tables(const gd_plan* p, int n, int q)
@@ -456,7 +474,9 @@ int gd_plan_create(const gd_desc* desc, gd_plan** out)
   gd::copy_sep(p, nullptr, p->sigma_host, 1.0);
   for (int d = 0; d < D; ++d) gd::copy_sep(p, nullptr, p->beta_host[d], 0.0);
   gd::copy_sep(p, nullptr, p->bc_host, 0.0);
-  const int rc = gd::upload_fields(p);
+  int rc = gd::upload_fields(p);
+  if (rc == GD_OK) rc = gd::k3k::refresh(p);
+  if (rc == GD_OK) rc = gd::pa::refresh(p);
   if (rc != GD_OK) {
     delete p;
     return rc;
@@ -497,8 +517,7 @@ int gd_plan_set_sigma(gd_plan* plan, const gd_separable* sigma)
   int rc = gd::check_sep(plan, sigma, "sigma", false);
   if (rc) return rc;
   gd::copy_sep(plan, sigma, plan->sigma_host, 1.0);
-  ++plan->coef_version;
-  return gd::upload_fields(plan);
+  return gd::commit_coefficients(plan, true);
 }
 
 int gd_plan_set_beta_separable(gd_plan* plan, const gd_separable* beta)
@@ -511,8 +530,7 @@ int gd_plan_set_beta_separable(gd_plan* plan, const gd_separable* beta)
   for (int d = 0; d < plan->dim; ++d) gd::copy_sep(plan, &beta[d], plan->beta_host[d], 0.0);
   plan->beta_kind = GD_BETA_SEPARABLE;
   plan->beta_set = true;
-  ++plan->coef_version;
-  return gd::upload_fields(plan);
+  return gd::commit_coefficients(plan, true);
 }
 
 int gd_plan_set_beta_affine(gd_plan* plan, const double* A, const double* c)
@@ -525,8 +543,7 @@ int gd_plan_set_beta_affine(gd_plan* plan, const double* A, const double* c)
   }
   plan->beta_kind = GD_BETA_AFFINE;
   plan->beta_set = true;
-  ++plan->coef_version;
-  return GD_OK;
+  return gd::commit_coefficients(plan, false);
 }
 
 int gd_plan_set_beta_pointwise_f64(gd_plan* plan, const double* beta_cell, const double* const* beta_face)
@@ -540,8 +557,7 @@ int gd_plan_set_beta_pointwise_f64(gd_plan* plan, const double* beta_cell, const
   }
   plan->beta_kind = GD_BETA_POINTWISE;
   plan->beta_set = true;
-  ++plan->coef_version;
-  return GD_OK;
+  return gd::commit_coefficients(plan, false);
 }
 
 int gd_plan_set_bc(gd_plan* plan, int32_t mode, const gd_separable* g)
@@ -552,8 +568,7 @@ int gd_plan_set_bc(gd_plan* plan, int32_t mode, const gd_separable* g)
   if (rc) return rc;
   plan->bc_mode = mode;
   gd::copy_sep(plan, g, plan->bc_host, 0.0);
-  ++plan->coef_version;
-  return gd::upload_fields(plan);
+  return gd::commit_coefficients(plan, true);
 }
 
 int gd_malloc(void** ptr, size_t bytes)

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib as L
 from .discretization import FiniteElementSpace, IntegrationRule, make_desc
-from .fields import AffineField, PointwiseField, SeparableField, VectorField
+from .fields import AffineField, PointwiseField, SeparableField, VectorField, face_counts
 
 
 def _stream():
@@ -30,6 +30,14 @@ def _check_vec(t, n, dtype, name):
         raise TypeError(f"{name} must be a CUDA (HIP) torch tensor")
     if t.dtype != dtype or t.numel() != n or not t.is_contiguous():
         raise ValueError(f"{name}: expected contiguous {dtype} with {n} entries, got {t.dtype} {tuple(t.shape)}")
+
+
+def _overlap(a: torch.Tensor, b: torch.Tensor) -> bool:
+    """True when the device storage ranges of two contiguous tensors intersect."""
+    a0, b0 = a.data_ptr(), b.data_ptr()
+    a1 = a0 + a.numel() * a.element_size()
+    b1 = b0 + b.numel() * b.element_size()
+    return a0 < b1 and b0 < a1
 
 
 def _span(t, fes, name):
@@ -107,6 +115,20 @@ class Plan:
 
     def set_beta_pointwise(self, cell: torch.Tensor, faces: Sequence[torch.Tensor]):
         D = self.fes.dim
+        faces = list(faces)
+        if self.fes.padaptive:
+            raise ValueError("pointwise beta needs a uniform order and rule")
+        if len(faces) != D:
+            raise ValueError(f"pointwise beta needs one face array per axis ({D}), got {len(faces)}")
+        # sizes the kernels index: cell [E][D][q^D], face a [F_a][q^(D-1)]
+        q = self.q
+        want = self.fes.mesh.num_cells * D * q ** D
+        if cell.numel() != want:
+            raise ValueError(f"pointwise beta: cell array has {cell.numel()} values, expected E*D*q^D = {want}")
+        for a, (f, Fa) in enumerate(zip(faces, face_counts(self.fes.mesh))):
+            if f.numel() != Fa * q ** (D - 1):
+                raise ValueError(f"pointwise beta: face array {a} has {f.numel()} values, "
+                                 f"expected F_a*q^(D-1) = {Fa * q ** (D - 1)}")
         for t in [cell, *faces]:
             if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
                 raise ValueError("pointwise beta must be contiguous float64 device tensors")
@@ -317,7 +339,7 @@ class AdvectionFaceOperator:
         xs, ys = _span(x, self.plan.fes, "x"), _span(y, self.plan.fes, "y")
         _check_vec(xs, self.plan.num_dofs, torch.float64, "x")
         _check_vec(ys, self.plan.num_dofs, torch.float64, "y")
-        if xs.data_ptr() == ys.data_ptr():
+        if _overlap(xs, ys):
             raise ValueError("x and y must not overlap")
         L.check(L.lib().gd_adv_face_apply_f64(self.plan.handle, xs.data_ptr(), ys.data_ptr(), _stream()),
                 "gd_adv_face_apply_f64")
@@ -343,6 +365,11 @@ class AdvectionNonconformingFaceOperator:
         for t, n, name in ((xm, fm.num_dofs, "x"), (xp, fp.num_dofs, "x"), (ym, fm.num_dofs, "y"),
                            (yp, fp.num_dofs, "y")):
             _check_vec(t, n, torch.float64, name)
+        # the kernels read x while they add into y (the h-refined interface in
+        # one launch per subface position), so no part of x may share storage
+        # with a part of y
+        if any(_overlap(a, b) for a in (xm, xp) for b in (ym, yp)):
+            raise ValueError("x and y must not overlap")
         L.check(
             getattr(L.lib(), self._entry)(self.minus.handle, self.plus.handle, self.face, xm.data_ptr(),
                                           xp.data_ptr(), ym.data_ptr(), yp.data_ptr(), _stream()),

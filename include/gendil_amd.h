@@ -98,14 +98,21 @@ int gd_plan_axis_points(const gd_plan* plan, int32_t axis, double* out);
 /* 1D rule on [0,1]: GL (basis=1) or GLL (basis=0) points and weights. */
 int gd_rule_points(int32_t basis, int32_t npts, double* pts, double* wts);
 
-/* ---- coefficients (set once after create, before applies) -------------- */
+/* ---- coefficients -------------------------------------------------------
+ * Each setter first waits for all work on the device (hipDeviceSynchronize),
+ * so applies queued earlier on any stream use the previous coefficients, then
+ * rebuilds every device table the kernels read (synchronous copies).  Applies
+ * queued after it use the new coefficients.  Setters are set-up calls: the
+ * apply entry points themselves never allocate and never copy, and they are
+ * ordered only on the stream they are given. */
 /* mass density sigma(X) (massoperator.hpp:65-83) */
 int gd_plan_set_sigma(gd_plan* plan, const gd_separable* sigma);
 /* advection field beta(X), one separable per component (dim entries) */
 int gd_plan_set_beta_separable(gd_plan* plan, const gd_separable* beta);
 /* beta(X) = c + A X, A row-major dim x dim */
 int gd_plan_set_beta_affine(gd_plan* plan, const double* A, const double* c);
-/* per-point beta (device pointers, caller-owned, read at apply time):
+/* per-point beta (device pointers, caller-owned, read at apply time; the
+ * arrays must hold exactly these sizes, which the C-ABI cannot check):
  * cell [E][dim][q^dim]; face normal component for axis a:
  * [F_a][q^(dim-1)], F_a = prod_{d!=a} sizes[d] * (sizes[a] + !periodic[a]),
  * global face f_a between element f_a-1 (+ side) and f_a (- side). */
@@ -154,7 +161,8 @@ int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, double* y, gd_st
  * mesh glued to the opposite face of the plus plan's mesh: y_minus += and
  * y_plus -= the upwind fluxes on the interface, both sides interpolated to
  * the shared rule (both plans must have the same num_quad; their orders may
- * differ), geometry and beta from the minus plan.  Replaces
+ * differ), geometry and beta from the minus plan.  No input side may share
+ * storage with an output side (GD_ERR_INVALID).  Replaces
  * AdvectionNonconformingFaceOperator over CartesianIntermeshFaceConnectivity
  * <dim, face> (advectionoperator.hpp:578-676, 1210-1396; cartesianintermeshface
  * connectivity.hpp:36-180), built by MakeAdvectionFaceOperator(fes_minus,
@@ -172,7 +180,10 @@ int gd_adv_intermesh_apply_f64(const gd_plan* minus, const gd_plan* plus, int32_
  * NonconformingCartesianIntermeshFaceConnectivity<dim, face>
  * (nonconformingcartesianintermeshfaceconnectivity.hpp:73-243;
  * advectionoperator.hpp:596-676; facetquaddata.hpp:86-155).  Same rule on
- * both plans; accumulates. */
+ * both plans; accumulates over one launch per subface position, so it never
+ * runs in place: no input side may share storage with an output side
+ * (GD_ERR_INVALID).  The first call for a new refinement ratio builds the
+ * subface basis tables on the coarse plan (synchronous allocation and copy). */
 int gd_adv_hmesh_apply_f64(const gd_plan* coarse, const gd_plan* fine, int32_t face,
                            const double* x_coarse, const double* x_fine, double* y_coarse,
                            double* y_fine, gd_stream stream);

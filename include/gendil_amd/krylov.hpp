@@ -157,6 +157,9 @@ namespace detail {
 inline double dev_dot(int64_t n, const double* a, const double* b, double* scratch, gd_stream s)
 {
   check(gd_multidot_f64(n, 1, a, n, b, scratch, s), "gd_multidot_f64");
+  // gd_memcpy_d2h runs on the null stream, which is not ordered after a
+  // non-blocking stream: wait for the dot product on its own stream first
+  check(gd_stream_sync(s), "gd_stream_sync");
   double h = 0.0;
   check(gd_memcpy_d2h(&h, scratch, sizeof(double)), "gd_memcpy_d2h");
   return h;

@@ -11,7 +11,9 @@ import this package, and only as the checker: the product path never calls it.
 Parity status: gdo_apply pinned by golden vectors from _ref/gendil_ref
 (tests/golden); gdo_apply_transpose pinned by dense transposes and the
 dot-product identity; gdo_grad_beta pinned by central finite differences;
-the p-adaptive apply pinned by its uniform special case and by conservation;
+the p-adaptive apply pinned by GenDiL's own p-adaptivity (uniform-order blocks
+coupled by its intermesh face operator, tests/golden/padapt*), by its uniform
+special case and by conservation;
 bicgstab pinned by GenDiL's own BiCGSTAB_no_alloc run through _ref/gendil_ref
 (tests/golden/bicgstab3d_c2_p1q2_bc0.npz) and by dense solves; the face-centric,
 intermesh and h-refined face operators (advection_faces, hmesh_operator) pinned
@@ -70,6 +72,7 @@ class GdoProblem(C.Structure):
         ("elem_nq", C.POINTER(C.c_int8)),
         ("dof_offset", C.POINTER(C.c_int64)),
         ("num_threads", C.c_int32),
+        ("plant", C.c_int32),
     ]
 
 
@@ -86,6 +89,7 @@ def lib():
         pd = C.POINTER(C.c_double)
         h.gdo_apply.argtypes = [P, pd, pd]
         h.gdo_apply_transpose.argtypes = [P, pd, pd]
+        h.gdo_apply_elements.argtypes = [P, pd, pd, C.POINTER(C.c_int64), C.c_int64]
         h.gdo_grad_beta.argtypes = [P, pd, pd, pd, C.POINTER(pd)]
         h.gdo_points.argtypes = [C.c_int32, C.c_int32, pd, pd]
         h.gdo_axis_points.argtypes = [P, C.c_int32, pd]
@@ -109,7 +113,8 @@ def points(basis: int, n: int):
 class Problem:
     """Oracle view of a discretization + coefficients (numpy, host)."""
 
-    def __init__(self, fes, ir, mass_coef=0.0, adv_coef=0.0, sigma=None, adv=None, bc=None, num_threads=0):
+    def __init__(self, fes, ir, mass_coef=0.0, adv_coef=0.0, sigma=None, adv=None, bc=None, num_threads=0,
+                 plant=0):
         from gendil_amd import fields as F
         from gendil_amd._lib import BC_INFLOW, BC_NONE
 
@@ -129,6 +134,7 @@ class Problem:
         p.mass_coef = mass_coef
         p.adv_coef = adv_coef
         p.num_threads = num_threads
+        p.plant = plant  # a planted mistake (gd_oracle.h), for tests of the fixtures only
         self._keep = []
         if sigma is None:
             sigma = F.SeparableField.constant(1.0)
@@ -193,6 +199,24 @@ class Problem:
         if rc:
             raise RuntimeError(f"gdo_apply failed ({rc})")
         return y
+
+    def apply_elements(self, x: np.ndarray, elems) -> np.ndarray:
+        """The forward apply at the DoFs of the listed elements only; returns
+        their outputs concatenated in list order."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        el = np.ascontiguousarray(np.asarray(elems, dtype=np.int64).reshape(-1))
+        y = np.zeros(self.num_dofs)
+        rc = lib().gdo_apply_elements(C.byref(self.p), _dp(x), _dp(y), el.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      el.size)
+        if rc:
+            raise RuntimeError(f"gdo_apply_elements failed ({rc})")
+        return y[self.element_dofs(el)]
+
+    def element_dofs(self, elems) -> np.ndarray:
+        """Indices of the DoFs of the listed elements, concatenated in list order."""
+        off = self.fes.dof_offsets()
+        el = np.asarray(elems, dtype=np.int64)
+        return np.concatenate([np.arange(off[e], off[e + 1]) for e in el]) if el.size else np.zeros(0, np.int64)
 
     def apply_transpose(self, r: np.ndarray) -> np.ndarray:
         r = np.ascontiguousarray(r, dtype=np.float64)

@@ -23,6 +23,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #ifdef _OPENMP
@@ -648,13 +649,39 @@ void element_apply(const Problem& P, Mode mode, int64_t e, const double* x, doub
       unravel(nb, P.M, D, nsub);
       const int qn = P.nq(nb);
       qf = qn > q ? qn : q;  // p-adaptive face rule: max of the two cell rules
+      if (p->plant == 1) qf = qn < q ? qn : q;
       Tn = &cache().get(p->basis, P.order(nb) + 1, qf);
     }
     const Tables& Tf = cache().get(p->basis, n, qf);
     const FaceIter it(D, a, qf);
     std::vector<double> own(it.count), nbv(it.count), flux(it.count);
     trace(Tf, D, a, sign > 0, ue, own.data(), tmp);
-    if (!boundary) trace(*Tn, D, a, sign < 0, x + P.dof0(nb), nbv.data(), tmp);
+    if (!boundary) {
+      const double* unb = x + P.dof0(nb);
+      std::vector<double> swapped;
+      if (p->plant == 2 && P.order(nb) != ord && D >= 3) {
+        int t[2] = {-1, -1};
+        for (int d = 0, k = 0; d < D && k < 2; ++d)
+          if (d != a) t[k++] = d;
+        const int nn = P.order(nb) + 1;
+        const int64_t ndn = ipow(nn, D);
+        swapped.resize(ndn);
+        for (int64_t i = 0; i < ndn; ++i) {
+          int idx[kMaxDim];
+          int64_t r = i;
+          for (int d = 0; d < D; ++d) {
+            idx[d] = (int)(r % nn);
+            r /= nn;
+          }
+          std::swap(idx[t[0]], idx[t[1]]);
+          int64_t j = 0;
+          for (int d = D - 1; d >= 0; --d) j = j * nn + idx[d];
+          swapped[j] = unb[i];
+        }
+        unb = swapped.data();
+      }
+      trace(*Tn, D, a, sign < 0, unb, nbv.data(), tmp);
+    }
     int kf[kMaxDim];
     for (int64_t i = 0; i < it.count; ++i) {
       it.sub(i, kf);
@@ -765,6 +792,22 @@ int gdo_apply(const gdo_problem* p, const double* x, double* y)
 {
   Problem P(p);
   parallel_elements(P, [&](int64_t e, Work& W) { element_apply(P, Mode::Forward, e, x, y, W); });
+  return 0;
+}
+
+int gdo_apply_elements(const gdo_problem* p, const double* x, double* y, const int64_t* elems, int64_t count)
+{
+  Problem P(p);
+  for (int64_t i = 0; i < count; ++i)
+    if (elems[i] < 0 || elems[i] >= P.E) return 1;
+  const int64_t sz = max_local(P);
+#pragma omp parallel num_threads(P.p->num_threads > 0 ? P.p->num_threads : omp_get_max_threads())
+  {
+    Work W;
+    W.ensure(sz);
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t i = 0; i < count; ++i) element_apply(P, Mode::Forward, elems[i], x, y, W);
+  }
   return 0;
 }
 

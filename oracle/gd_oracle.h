@@ -67,6 +67,13 @@ typedef struct gdo_problem {
   const int8_t* elem_nq;
   const int64_t* dof_offset;
   int32_t num_threads;  /* 0 = OpenMP default */
+  /* Planted mistakes, for tests that show the GenDiL fixtures catch them
+   * (0 = none): 1 = a face between elements of different orders integrated
+   * with the smaller of the two cell rules instead of the larger; 2 = the
+   * neighbour's trace on such a face taken from its node slab with two
+   * tangential node axes exchanged (D >= 3).  The first is the same on both
+   * sides of a face, so conservation cannot see it. */
+  int32_t plant;
 } gdo_problem;
 
 int gdo_version(void);
@@ -78,6 +85,9 @@ int gdo_axis_points(const gdo_problem* p, int32_t axis, double* out);
 int64_t gdo_num_dofs(const gdo_problem* p);
 /* y = mass_coef*M x + adv_coef*A x (element-centric, overwrite). */
 int gdo_apply(const gdo_problem* p, const double* x, double* y);
+/* The forward apply on a subset of elements: writes y only at the DoFs of
+ * elems[0..count) (for checks of a large apply on a sample of its elements). */
+int gdo_apply_elements(const gdo_problem* p, const double* x, double* y, const int64_t* elems, int64_t count);
 /* z = (linear part of mass_coef*M + adv_coef*A)^T r. */
 int gdo_apply_transpose(const gdo_problem* p, const double* r, double* z);
 /* For pointwise beta: dL/dbeta_cell [E][D][nq^D] and dL/dbeta_face[a]

@@ -4,12 +4,8 @@
 //
 // This file is test infrastructure: it is compiled against the reference
 // headers in place (see oracle/Makefile) into oracle/_ref/ and is never
-// linked into the product library.
-//
-// Reference call pattern followed:
-//   tests/integration/gendil/test-mass-3d-generic.cpp:83-150
-//   tests/integration/gendil/test-advection-3d-generic.cpp:84-170
-//   benchmarks/include/benchmark-common.hpp:276-295 (TimeOperator)
+// linked into the product library.  Shared pieces are in ref_common.hpp; the
+// p-adaptive block composites are in ref_padapt.cpp.
 //
 // Usage:
 //   gendil_ref fixture <op> <D> <P> <Q> <basis> <n> <periodic> <coef> <bc> <seed> <out.bin>
@@ -19,194 +15,21 @@
 //           P glued to an n^3 plus mesh of order PP, q shared; vector [minus|plus])
 //           | hmesh_<F>_<R> (GLL order P: face F of a coarse n^D mesh glued to
 //           a mesh R times finer along the face; vector [coarse|fine])
+//           | padapt_<F>_<PP> (two GLL blocks of orders P and PP, each with its
+//           own element-centric operator on q = p+1 points, coupled by the
+//           intermesh face operator on face F with q = max(P, PP) + 1; see
+//           ref_padapt.cpp)
 //   basis : L (Gauss-Lobatto nodes) | G (Gauss-Legendre nodes)
-//   coef  : mass -> one | poly ;  adv -> c2 | bubble | const | trig | phase | yzx
+//   coef  : mass -> one | poly ;  adv -> c2 | bubble | const | trig | phase | yzx | ones
 //   bc    : none (boundary faces skipped) | zero | poly
 // fixture output: int64 ndofs, then x[ndofs] (FillRandom(seed)), then y[ndofs].
 
-#include <gendil/gendil.hpp>
-
-#include <array>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <string>
-#include <utility>
+#include "ref_common.hpp"
 
 using namespace gendil;
+using namespace gdref;
 
 namespace {
-
-struct Args
-{
-   std::string mode, op, basis, coef, bc;
-   int D = 3, P = 3, Q = 4, n = 4, periodic = 0;
-   unsigned seed = 12345;
-   std::string out;
-   int pairs = 1;
-};
-
-constexpr double kPi = 3.14159265358979323846;
-
-template < int D >
-void advection_field( const std::string & kind, const std::array< Real, D > & X, Real ( &v )[D] )
-{
-   if ( kind == "c2" )
-   {
-      if constexpr ( D == 3 )
-      {
-         v[0] = 1.0 + 0.5 * std::sin( 2.0 * kPi * X[1] );
-         v[1] = 0.5 + 0.5 * std::cos( 2.0 * kPi * X[2] );
-         v[2] = 0.25 + 0.25 * std::sin( 2.0 * kPi * X[0] );
-      }
-   }
-   else if ( kind == "bubble" )
-   {
-      for ( int d = 0; d < D; ++d ) v[d] = -X[d] * ( 1.0 - X[d] );
-   }
-   else if ( kind == "const" )
-   {
-      Real c = 1.0;
-      for ( int d = 0; d < D; ++d ) { v[d] = c; c *= -0.5; }
-   }
-   else if ( kind == "trig" )
-   {
-      for ( int d = 0; d < D; ++d ) v[d] = 0.5 + std::sin( 2.0 * kPi * X[( d + 1 ) % D] );
-   }
-   else if ( kind == "yzx" )
-   {
-      // tests/integration/gendil/test-advection-p-adaptivity.cpp:62-71
-      if constexpr ( D == 3 )
-      {
-         v[0] = X[1];
-         v[1] = X[2];
-         v[2] = X[0];
-      }
-   }
-   else if ( kind == "phase" )
-   {
-      if constexpr ( D == 6 )
-      {
-         v[0] = X[3];
-         v[1] = X[4];
-         v[2] = X[5];
-         v[3] = 0.1;
-         v[4] = 0.0;
-         v[5] = 0.0;
-      }
-   }
-   else
-   {
-      std::fprintf( stderr, "unknown advection field %s\n", kind.c_str() );
-      std::exit( 2 );
-   }
-}
-
-template < int D >
-Real mass_density( const std::string & kind, const std::array< Real, D > & X )
-{
-   if ( kind == "one" ) return 1.0;
-   if ( kind == "poly" )
-   {
-      if constexpr ( D >= 2 ) return 1.0 + X[0] * X[1] + X[D - 1] * X[D - 1];
-      else return 1.0 + X[0] * X[0];
-   }
-   std::fprintf( stderr, "unknown mass density %s\n", kind.c_str() );
-   std::exit( 2 );
-}
-
-template < int D >
-Real boundary_value( const std::string & kind, const std::array< Real, D > & X )
-{
-   if ( kind == "zero" ) return 0.0;
-   if ( kind == "poly" )
-   {
-      if constexpr ( D >= 2 ) return 1.0 + X[0] - 0.5 * X[1];
-      else return 1.0 + X[0];
-   }
-   std::fprintf( stderr, "unknown boundary field %s\n", kind.c_str() );
-   std::exit( 2 );
-}
-
-template < int D, int P, int Q, bool Lobatto, size_t... Is >
-auto make_space( const Args & a, std::index_sequence< Is... > )
-{
-   std::array< GlobalIndex, D > sizes;
-   std::array< Real, D > h;
-   Point< D > origin;
-   for ( int d = 0; d < D; ++d )
-   {
-      sizes[d] = a.n;
-      h[d] = 1.0 / a.n;
-      origin[d] = 0.0;
-   }
-   CartesianMesh< D > mesh( sizes, h, origin, a.periodic != 0 );
-   FiniteElementOrders< ( (void)Is, P )... > orders;
-   if constexpr ( Lobatto )
-   {
-      auto fe = MakeLobattoFiniteElement( orders );
-      return MakeFiniteElementSpace( mesh, fe );
-   }
-   else
-   {
-      auto fe = MakeLegendreFiniteElement( orders );
-      return MakeFiniteElementSpace( mesh, fe );
-   }
-}
-
-template < int Q, size_t... Is >
-auto make_rule( std::index_sequence< Is... > )
-{
-   IntegrationRuleNumPoints< ( (void)Is, Q )... > nq;
-   return MakeIntegrationRule( nq );
-}
-
-template < typename Op >
-double time_pairs( const Op & op, Vector & x, Vector & y, int pairs )
-{
-   // warm-up pair, then timed pairs (TimeOperator pattern)
-   op( x, y );
-   op( y, x );
-   const auto t0 = std::chrono::steady_clock::now();
-   for ( int i = 0; i < pairs; ++i )
-   {
-      op( x, y );
-      op( y, x );
-   }
-   const auto t1 = std::chrono::steady_clock::now();
-   return std::chrono::duration< double >( t1 - t0 ).count() / ( 2.0 * pairs );
-}
-
-template < typename Op >
-int finish( const Args & a, const Op & op, Integer ndofs, Integer nelem )
-{
-   Vector x( ndofs );
-   Vector y( ndofs );
-   FillRandom( x, a.seed );
-   y = 0.0;
-   if ( a.mode == "fixture" )
-   {
-      op( x, y );
-      const Real * xd = x.ReadHostData();
-      const Real * yd = y.ReadHostData();
-      FILE * f = std::fopen( a.out.c_str(), "wb" );
-      if ( !f ) { std::perror( "fopen" ); return 3; }
-      const long long n = ndofs;
-      std::fwrite( &n, sizeof( n ), 1, f );
-      std::fwrite( xd, sizeof( Real ), ndofs, f );
-      std::fwrite( yd, sizeof( Real ), ndofs, f );
-      std::fclose( f );
-      return 0;
-   }
-   // bench: the two vectors are swapped each apply, as in TimeOperator; scale
-   // the input down so repeated applies stay finite.
-   const double t = time_pairs( op, x, y, a.pairs );
-   std::printf( "{\"seconds_per_apply\": %.9e, \"dofs\": %lld, \"elements\": %lld, \"dofs_per_s\": %.6e}\n",
-                t, (long long)ndofs, (long long)nelem, ndofs / t );
-   return 0;
-}
 
 // GenDiL's own BiCGSTAB_no_alloc (bicgstab.hpp:44) on K = M + dt A (unit mass
 // density, advection field `coef` with boundary field `bc`), b = FillRandom(seed),
@@ -278,8 +101,6 @@ int run( const Args & a )
    return 2;
 }
 
-using Runner = int ( * )( const Args & );
-
 // GenDiL's nonconforming face operator (advectionoperator.hpp:578-676,
 // 1210-1396) over CartesianIntermeshFaceConnectivity<3, F>: face F of an n^3
 // minus mesh of GLL order PM at the origin glued to an n^3 plus mesh of order
@@ -324,12 +145,6 @@ struct IntermeshEntry { int PM, PP, Q, F; Runner fn; };
 // shifted by one unit along the face normal, GLL order P on both sides, Q
 // points shared, vector [coarse | fine] (as
 // tests/integration/gendil/test-advection-h-adaptivity-2d.cpp:66-140).
-template < int P, size_t... Is >
-auto make_gll( std::index_sequence< Is... > )
-{
-   return MakeLobattoFiniteElement( FiniteElementOrders< ( (void)Is, P )... >{} );
-}
-
 template < int D, int P, int Q, int F, int R >
 int run_hmesh( const Args & a )
 {
@@ -461,6 +276,7 @@ int main( int argc, char ** argv )
    if ( a.mode == "fixture" || a.mode == "bicgstab" ) a.out = argv[12];
    else a.pairs = std::atoi( argv[12] );
    if ( a.op == "adv6dprod" ) return run_product( a );
+   if ( a.op.rfind( "padapt_", 0 ) == 0 ) return run_padapt_case( a );
    if ( a.op == "advface" )
    {
       for ( const CaseEntry & e : kFaceEntries )

@@ -277,3 +277,109 @@ def rel_l2(a, b) -> float:
     nb = np.linalg.norm(b)
     d = np.linalg.norm(a - b)
     return float(d / nb) if nb > 0 else float(d)
+
+
+def padapt_field(kind: str, D: int):
+    """Fields of the p-adaptive composites in the descriptor forms the
+    p-adaptive kernels take (affine or constant components)."""
+    if kind == "phase":
+        assert D == 6
+        return W.phase_space_6d_affine()
+    if kind == "yzx":
+        return beta_field("yzx", D)
+    if kind == "const":
+        return beta_field("const", D)
+    raise ValueError(kind)
+
+
+@dataclass(frozen=True)
+class PadaptCase:
+    """A p-adaptive block composite of oracle/ref_padapt.cpp (GenDiL's own
+    p-adaptivity: uniform-order blocks, each with its element-centric
+    operator on q = p + 1 points, coupled by the intermesh face operator on
+    q = max + 1 points).  The minus block (order PM) has n cells per axis at
+    the origin; the plus block (order PP) has n + 1 cells along the interface
+    axis, beyond face `face` of the minus block; h = 1/n; periodic along the
+    tangential axes when `periodic`.  GenDiL's vector is [minus | plus].
+
+    The same operator is one Cartesian mesh over both blocks with per-element
+    orders, which `layout()` builds together with the permutation between
+    the two DoF orderings."""
+
+    name: str
+    D: int
+    PM: int
+    PP: int
+    face: int
+    n: int
+    periodic: bool
+    coef: str
+    seed: int = 12345
+
+    @property
+    def harness_op(self):
+        return f"padapt_{self.face}_{self.PP}"
+
+    @property
+    def axis(self):
+        return self.face % self.D
+
+    def layout(self):
+        """(space, perm): the per-element-order space over the union of the
+        blocks, and perm with ours[perm[i]] = GenDiL's entry i."""
+        D, n, ax = self.D, self.n, self.axis
+        h = 1.0 / n
+        plus_first = self.face < D
+        gsizes = [2 * n + 1 if d == ax else n for d in range(D)]
+        origin = [0.0] * D
+        if plus_first:
+            origin[ax] = -(n + 1) * h
+        periodic = [bool(self.periodic) and d != ax for d in range(D)]
+        mesh = G.CartesianMesh(gsizes, [h] * D, origin, periodic)
+        sub = np.array(np.unravel_index(np.arange(mesh.num_cells), gsizes[::-1]))[::-1]  # x fastest
+        in_minus = sub[ax] >= n + 1 if plus_first else sub[ax] < n
+        orders = np.where(in_minus, self.PM, self.PP).astype(np.int8)
+        fes = G.MakePAdaptiveSpace(mesh, orders, orders + 1)
+        offs = fes.dof_offsets()
+        strides = np.cumprod([1] + gsizes[:-1])
+        parts = []
+        blocks = (
+            ([n] * D, (n + 1) if plus_first else 0, self.PM),
+            ([n + 1 if d == ax else n for d in range(D)], 0 if plus_first else n, self.PP),
+        )
+        for bsizes, shift, p in blocks:
+            nb = int(np.prod(bsizes))
+            bsub = np.array(np.unravel_index(np.arange(nb), bsizes[::-1]))[::-1]
+            bsub[ax] += shift
+            g = (bsub * strides[:, None]).sum(0)
+            nd = (p + 1) ** D
+            parts.append((offs[g][:, None] + np.arange(nd)[None, :]).reshape(-1))
+        return fes, np.concatenate(parts)
+
+    def field(self):
+        return padapt_field(self.coef, self.D)
+
+
+PADAPT_CASES = [
+    # +x interface, every ordered pair of orders 1..3, beta = (y, z, x) (the
+    # p-adaptivity test's field), tangential axes periodic
+    *[PadaptCase(f"padapt3d_yzx_f3_p{a}p{b}", 3, a, b, 3, 3, True, "yzx")
+      for a, b in ((1, 2), (2, 1), (1, 3), (3, 1), (2, 3), (3, 2))],
+    # -y interface, every ordered pair, no periodic axes
+    *[PadaptCase(f"padapt3d_yzx_f1_p{a}p{b}", 3, a, b, 1, 3, False, "yzx")
+      for a, b in ((1, 2), (2, 1), (1, 3), (3, 1), (2, 3), (3, 2))],
+    # the other orientation of each axis, and the z axis; a constant field of
+    # mixed signs
+    PadaptCase("padapt3d_yzx_f0_p1p3", 3, 1, 3, 0, 3, True, "yzx"),
+    PadaptCase("padapt3d_const_f0_p3p2", 3, 3, 2, 0, 3, False, "const"),
+    PadaptCase("padapt3d_const_f4_p2p1", 3, 2, 1, 4, 3, True, "const"),
+    PadaptCase("padapt3d_yzx_f4_p3p1", 3, 3, 1, 4, 3, False, "yzx"),
+    PadaptCase("padapt3d_yzx_f5_p2p3", 3, 2, 3, 5, 3, True, "yzx"),
+    PadaptCase("padapt3d_const_f2_p1p2", 3, 1, 2, 2, 3, True, "const"),
+    # phase space (beta = (X3, X4, X5, 0.1, 0, 0)): split along a velocity
+    # axis and along a space axis
+    PadaptCase("padapt6d_phase_f9_p1p2", 6, 1, 2, 9, 2, True, "phase"),
+    PadaptCase("padapt6d_phase_f0_p2p1", 6, 2, 1, 0, 2, True, "phase"),
+]
+
+PADAPT_BY_NAME = {c.name: c for c in PADAPT_CASES}

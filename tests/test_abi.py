@@ -94,3 +94,38 @@ def test_product_mesh_is_the_concatenated_cartesian_mesh():
     assert tuple(sub) == head + tail
     with pytest.raises(ValueError):
         G.MakeCartesianProductMesh()
+
+
+def test_pointwise_beta_sizes_are_checked_before_any_device_call():
+    # a wrong cell or face array size is refused on the host: the kernels
+    # would otherwise read (and grad_beta write) out of bounds
+    import torch
+
+    from gendil_amd.operators import Plan
+
+    mesh = G.CartesianMesh.unit(3, 2, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(1))
+    plan = Plan.__new__(Plan)  # no gd_plan behind it: nothing may reach the C-ABI
+    plan.fes, plan.q = fes, 2
+    E, q = mesh.num_cells, 2
+    faces = [torch.zeros(n * q * q, dtype=torch.float64) for n in G.face_counts(mesh)]
+    cell = torch.zeros(E * 3 * q ** 3, dtype=torch.float64)
+    with pytest.raises(ValueError, match="cell array"):
+        plan.set_beta_pointwise(torch.zeros(E * 3 * 27, dtype=torch.float64), faces)
+    with pytest.raises(ValueError, match="face array 1"):
+        plan.set_beta_pointwise(cell, [faces[0], torch.zeros(faces[1].numel() + 4, dtype=torch.float64), faces[2]])
+    with pytest.raises(ValueError, match="one face array per axis"):
+        plan.set_beta_pointwise(cell, faces[:2])
+    # right sizes, but host tensors: refused before the C-ABI as well
+    with pytest.raises(ValueError, match="device tensors"):
+        plan.set_beta_pointwise(cell, faces)
+
+
+def test_face_operator_wrappers_reject_overlapping_storage():
+    from gendil_amd.operators import _overlap
+    import torch
+
+    v = torch.zeros(100, dtype=torch.float64)
+    assert _overlap(v[:50], v[49:])
+    assert not _overlap(v[:50], v[50:])
+    assert _overlap(v, v[10:20])

@@ -282,3 +282,28 @@ def test_cg_matches_host_recurrence_iterations():
     res = S.cg(M, b, xd, rel_tol=1e-10)
     assert res.success and abs(res.iterations - it) <= 1, (res.iterations, it)
     assert rel_l2(xd.cpu().numpy(), x) <= 1e-8
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed", "cg"])
+def test_solves_on_a_side_stream(solver):
+    # every host read of a device dot product (||b||, restart and true
+    # residuals) must wait for the solver's own stream, not the null stream
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        if solver == "cg":
+            mesh = G.CartesianMesh.unit(3, 6, False)
+            fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(2))
+            K = G.MakeMassFiniteElementOperator(fes, G.MakeIntegrationRule(3))
+            Ko = oracle.Problem(fes, G.MakeIntegrationRule(3), mass_coef=1.0)
+            b = torch.as_tensor(G.fill_random(K.num_dofs, 3), device="cuda")
+        else:
+            K, b, Ko = implicit_step(8)
+        x = torch.zeros_like(b)
+        fn = {"fp64": S.gmres, "mixed": S.gmres_mixed, "cg": S.cg}[solver]
+        res = fn(K, b, x, rel_tol=1e-10)
+    side.synchronize()
+    assert res.success, res
+    bb = b.cpu().numpy()
+    rel = np.linalg.norm(bb - Ko.apply(x.cpu().numpy())) / np.linalg.norm(bb)
+    print(f"{solver} on a side stream: {res.iterations} iterations, oracle residual {rel:.3e}")
+    assert rel <= 2e-10
