@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES comparison")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--no-padapt", action="store_true", help="skip the 6D p-adaptive apply")
+    ap.add_argument("--no-padapt", action="store_true",
+                    help="skip the 6D p-adaptive, product-mesh, adjoint and face-operator entries")
+    ap.add_argument("--no-mass", action="store_true", help="skip the 32^3 mass apply")
     ap.add_argument("--face-ops-only", action="store_true",
                     help="only time the face-centric operators (one JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -79,22 +81,69 @@ def time_applies(op, xs, ys, steps, warmup):
     return start.elapsed_time(end) / steps  # ms per apply
 
 
-def cpu_baseline(n):
+def host_cpu():
+    """CPU model, physical cores of this host and of this process's CPU set,
+    and the OpenMP threads the CPU baseline uses: the CPU share this process
+    is given (OMP_NUM_THREADS where the environment sets it, else the
+    physical cores of its CPU set)."""
+    model, phys_all, phys_mine = "unknown", set(), set()
+    try:
+        mine = os.sched_getaffinity(0)
+    except AttributeError:
+        mine = set(range(os.cpu_count() or 1))
+    try:
+        cpu = pid = cid = None
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().splitlines() + [""]:
+                if not line.strip():
+                    if cpu is not None:
+                        phys_all.add((pid, cid))
+                        if cpu in mine:
+                            phys_mine.add((pid, cid))
+                    cpu = pid = cid = None
+                    continue
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    cpu = int(v)
+                elif k == "model name":
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    cid = v
+    except OSError:
+        pass
+    cores_mine = len(phys_mine) or len(mine)
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = int(env) if env and env.isdigit() and int(env) > 0 else cores_mine
+    return {"model": model, "host_physical_cores": len(phys_all) or None, "process_cpus": len(mine),
+            "process_physical_cores": cores_mine, "threads": threads}
+
+
+def cpu_baseline(n, pairs=5):
+    """GenDiL's own OpenMP apply (SerialKernelConfiguration, compiled from its
+    headers into oracle/_ref/gendil_ref) on the 64^3 workload, timed with its
+    TimeOperator pattern (benchmark-common.hpp:276-295): one warm-up pair,
+    then `pairs` timed pairs of applies."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    cpu = host_cpu()
+    threads = cpu["threads"]
     env = {"OMP_NUM_THREADS": str(threads), "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}
     if oracle.ref_available():
-        r = oracle.ref_bench("adv", 3, 3, 4, "L", n, True, "c2", "none", 12345, 1, env=env)
+        r = oracle.ref_bench("adv", 3, 3, 4, "L", n, True, "c2", "none", 12345, pairs, env=env)
         return {
             "value": r["dofs_per_s"] / 1e9,
             "unit": "GDoF/s",
             "cores": threads,
             "kind": "reference",
-            "sample": f"GenDiL SerialKernelConfiguration apply (OpenMP, {threads} threads) compiled from its "
-            f"headers, full {n}^3 p=3 C2 workload, 1 warm-up + 1 timed apply pair",
+            "cpu": cpu,
+            "sample": f"GenDiL SerialKernelConfiguration apply (OpenMP, {threads} threads, OMP_PROC_BIND=close, "
+            f"OMP_PLACES=cores) compiled from its headers, full {n}^3 p=3 workload, 1 warm-up + {pairs} timed "
+            f"apply pairs",
         }
-    # fallback: the project's CPU restatement on a 32^3 sample
+    # the oracle's restatement where GenDiL's binary is absent, on a 32^3 sample
     import numpy as np
 
     import gendil_amd as G
@@ -107,15 +156,61 @@ def cpu_baseline(n):
     x = G.fill_random(P.num_dofs, 12345)
     P.apply(x)
     t0 = time.perf_counter()
-    P.apply(x)
-    t = time.perf_counter() - t0
+    for _ in range(2 * pairs):
+        P.apply(x)
+    t = (time.perf_counter() - t0) / (2 * pairs)
     return {
         "value": P.num_dofs / t / 1e9,
         "unit": "GDoF/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU restatement (oracle) of GenDiL's apply, {m}^3 p=3 C2 workload, 1 timed apply",
+        "cpu": cpu,
+        "sample": f"CPU restatement (oracle) of GenDiL's apply, {m}^3 p=3 workload, {2 * pairs} timed applies",
     }
+
+
+def mass_32(steps, warmup, n=32, pairs=10):
+    """The mass apply y = M x on the 32^3 mesh (GLL p=3, 4 Gauss points,
+    sigma = 1, 2.1M DoFs, FP64): `pairs` input/output pairs rotate (16.8 MB
+    each, 336 MB in all), so the 256 MB Infinity Cache cannot hold the working
+    set; algorithmic bytes 16 per DoF.  Beside it GenDiL's own OpenMP mass
+    apply where it is built."""
+    import torch
+
+    import gendil_amd as G
+
+    fes = G.MakeFiniteElementSpace(G.CartesianMesh.unit(3, n, False), G.MakeLobattoFiniteElement(3))
+    op = G.MakeMassFiniteElementOperator(fes, G.MakeIntegrationRule(4))
+    N = op.num_dofs
+    x0 = torch.as_tensor(G.fill_random(N, 12345), device="cuda")
+    xs = [x0 * (1.0 - 0.01 * i) for i in range(pairs)]
+    ys = [torch.empty_like(x0) for _ in range(pairs)]
+    ms = time_applies(op, xs, ys, steps, warmup)
+    t = ms * 1e-3
+    out = {
+        "workload": f"mass apply, 3D {n}^3, GLL p=3, 4 GL pts/dim, sigma = 1, FP64, {pairs} rotating vector pairs",
+        "dofs": N,
+        "value": N / t / 1e9,
+        "unit": "GDoF/s",
+        "ms_per_step": ms,
+        "variant": op.plan.query_variant(torch.float64, 1.0, 0.0),
+        "roofline": {"bound": "hbm", "achieved": 16.0 * N / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": 16.0 * N / t / 1e9 / HBM_PEAK_GBS},
+    }
+    del xs, ys
+    try:
+        import oracle
+
+        if oracle.ref_available():
+            threads = host_cpu()["threads"]
+            r = oracle.ref_bench("mass", 3, 3, 4, "L", n, False, "one", "none", 12345, 5,
+                                 env={"OMP_NUM_THREADS": str(threads), "OMP_PROC_BIND": "close",
+                                      "OMP_PLACES": "cores"})
+            out["cpu_reference"] = {"value": r["dofs_per_s"] / 1e9, "unit": "GDoF/s", "cores": threads,
+                                    "kind": "reference", "sample": "1 warm-up + 5 timed apply pairs"}
+    except Exception as e:  # the GPU figure stands on its own
+        out["cpu_reference"] = {"error": str(e)[:200]}
+    return out
 
 
 def solve_comparison():
@@ -227,7 +322,13 @@ def padapt_6d(steps, warmup, n=8):
     y = ys[(steps - 1) % 3].double()
     cons = float(y.sum().abs() / y.abs().sum())
     counts = [int((fes.elem_order == p).sum()) for p in (1, 2, 3)]
+    # error of the FP32 apply against the FP64 apply of the same input
+    xlast = xs[(steps - 1) % 3]
     del xs, ys
+    y64 = torch.empty(N, dtype=torch.float64, device="cuda")
+    op(xlast.double(), y64)
+    err = float((y - y64).norm() / y64.norm())
+    del y64, y, xlast
     variant = op.plan.query_variant(torch.float32, 0.0, 1.0)
     compute = None
     if variant == 2:
@@ -256,6 +357,7 @@ def padapt_6d(steps, warmup, n=8):
         "autotuned_variant": chosen,
         "variant": variant,
         "conservation_sum_over_abs_sum": cons,
+        "err_vs_fp64": err,
     }
 
 
@@ -290,9 +392,10 @@ def product_6d(steps, warmup, n=5):
         import oracle
 
         if oracle.ref_available():
-            threads = min(16, os.cpu_count() or 1)
-            r = oracle.ref_bench("adv6dprod", 6, 1, 3, "G", n, False, "ones", "none", 12345, 1,
-                                 env={"OMP_NUM_THREADS": str(threads)})
+            threads = host_cpu()["threads"]
+            r = oracle.ref_bench("adv6dprod", 6, 1, 3, "G", n, False, "ones", "none", 12345, 5,
+                                 env={"OMP_NUM_THREADS": str(threads), "OMP_PROC_BIND": "close",
+                                      "OMP_PLACES": "cores"})
             out["cpu_reference"] = {"value": r["dofs_per_s"] / 1e9, "unit": "GDoF/s", "cores": threads,
                                     "kind": "reference"}
     except Exception as e:  # the GPU figure stands on its own
@@ -400,9 +503,10 @@ def face_ops(steps, warmup, n=64, p=3):
         import oracle
 
         if oracle.ref_available():
-            threads = min(16, os.cpu_count() or 1)
-            r = oracle.ref_bench("advface", 3, p, p + 1, "L", n, False, "c2", "none", 12345, 1,
-                                 env={"OMP_NUM_THREADS": str(threads)})
+            threads = host_cpu()["threads"]
+            r = oracle.ref_bench("advface", 3, p, p + 1, "L", n, False, "c2", "none", 12345, 5,
+                                 env={"OMP_NUM_THREADS": str(threads), "OMP_PROC_BIND": "close",
+                                      "OMP_PLACES": "cores"})
             out["interior"]["cpu_reference"] = {"value": r["dofs_per_s"] / 1e9, "unit": "GDoF/s", "cores": threads,
                                                 "kind": "reference"}
     except Exception as e:  # the GPU figure stands on its own
@@ -496,14 +600,23 @@ def main():
         x32 = [x.float() for x in xs]
         y32 = [torch.empty_like(x32[0]) for _ in range(3)]
         ms32 = time_applies(op, x32, y32, args.steps, args.warmup)
+        # error of the FP32 apply against the FP64 apply of the same input
+        y64 = torch.empty_like(xs[0])
+        op(x32[0].double(), y64)
+        op(x32[0], y32[0])
+        err32 = float((y32[0].double() - y64).norm() / y64.norm())
         result["fp32"] = {
             "value": N / (ms32 * 1e-3) / 1e9,
             "ms_per_step": ms32,
             "roofline_frac": 8.0 * N / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "err_vs_fp64": err32,
         }
-        del x32, y32
+        del x32, y32, y64
     del xs
     torch.cuda.empty_cache()
+    if not args.no_mass:
+        result["mass32"] = mass_32(args.steps, args.warmup)
+        torch.cuda.empty_cache()
     if not args.no_padapt:
         result["padapt6d"] = padapt_6d(max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()

@@ -52,13 +52,26 @@ def _compile(src: Path) -> tuple[Path, str]:
     return obj, msgs
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> Path:
-    """Compile all HIP/C++ sources for gfx950 and link the shared library."""
+def _up_to_date(src: Path) -> bool:
+    """The object exists and is newer than its source and every header."""
+    obj = BUILD / (src.name + ".o")
+    if not obj.exists():
+        return False
+    headers = list(CSRC.glob("*.hpp")) + list((ROOT / "include").rglob("*.h*"))
+    newest = max([src.stat().st_mtime] + [h.stat().st_mtime for h in headers])
+    return obj.stat().st_mtime > newest
+
+
+def build(jobs: int | None = None, verbose: bool = False, incremental: bool = False) -> Path:
+    """Compile all HIP/C++ sources for gfx950 and link the shared library
+    (with `incremental`, only sources whose object is out of date)."""
     BUILD.mkdir(exist_ok=True)
     srcs = _sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    todo = [s for s in srcs if not (incremental and _up_to_date(s))]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(_compile, srcs))
+        fresh = dict(ex.map(_compile, todo))
+    results = [(BUILD / (s.name + ".o"), fresh.get(BUILD / (s.name + ".o"), "")) for s in srcs]
     for obj, msgs in results:
         if verbose and msgs.strip():
             print(msgs, file=sys.stderr)
@@ -72,4 +85,4 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

@@ -20,6 +20,7 @@ const KernelSet* find_kernels(int D, int N, int Q)
     case 2: l = kernels_d2(); break;
     case 3: l = kernels_d3(); break;
     case 4: l = kernels_d4(); break;
+    case 5: l = kernels_d5(); break;
     case 6: l = kernels_d6(); break;
     default: break;
   }

@@ -26,6 +26,7 @@ KernelList kernels_d1();
 KernelList kernels_d2();
 KernelList kernels_d3();
 KernelList kernels_d4();
+KernelList kernels_d5();
 KernelList kernels_d6();
 
 const KernelSet* find_kernels(int D, int N, int Q);

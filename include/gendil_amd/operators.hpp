@@ -59,6 +59,70 @@ CartesianMesh<Dim> MakeUnitCartesianMesh(int64_t n, bool periodic)
   return m;
 }
 
+// n cells of size h from `origin` (Cartesian1DMesh, cartesian1dmesh.hpp)
+inline CartesianMesh<1> Cartesian1DMesh(double h, int64_t n, double origin = 0.0, bool periodic = false)
+{
+  return CartesianMesh<1>{{n}, {h}, {origin}, {periodic}};
+}
+
+// nx x ny square cells of size h (Cartesian2DMesh, cartesian2dmesh.hpp)
+inline CartesianMesh<2> Cartesian2DMesh(double h, int64_t nx, int64_t ny, std::array<double, 2> origin = {0.0, 0.0},
+                                        bool periodic = false)
+{
+  return CartesianMesh<2>{{nx, ny}, {h, h}, origin, {periodic, periodic}};
+}
+
+// nx x ny x nz cubic cells of size h (Cartesian3DMesh(h, nx, ny, nz),
+// cartesian3dmesh.hpp:32-47)
+inline CartesianMesh<3> Cartesian3DMesh(double h, int64_t nx, int64_t ny, int64_t nz,
+                                        std::array<double, 3> origin = {0.0, 0.0, 0.0}, bool periodic = false)
+{
+  return CartesianMesh<3>{{nx, ny, nz}, {h, h, h}, origin, {periodic, periodic, periodic}};
+}
+
+namespace detail {
+template <int D0, int D1>
+CartesianMesh<D0 + D1> concat(const CartesianMesh<D0>& a, const CartesianMesh<D1>& b)
+{
+  CartesianMesh<D0 + D1> m;
+  for (int d = 0; d < D0; ++d) {
+    m.sizes[d] = a.sizes[d];
+    m.h[d] = a.h[d];
+    m.origin[d] = a.origin[d];
+    m.periodic[d] = a.periodic[d];
+  }
+  for (int d = 0; d < D1; ++d) {
+    m.sizes[D0 + d] = b.sizes[d];
+    m.h[D0 + d] = b.h[d];
+    m.origin[D0 + d] = b.origin[d];
+    m.periodic[D0 + d] = b.periodic[d];
+  }
+  return m;
+}
+}  // namespace detail
+
+// Tensor product of Cartesian meshes (MakeCartesianProductMesh,
+// productmesh.hpp:154-225) as the Cartesian mesh over the concatenated axes.
+// The two index alike: the product numbers cell (head, tail) as
+// head + cells(head) * tail (productmesh.hpp:91-94), which unrolled over the
+// factors is the first-index-fastest order of the concatenated axes; a face
+// normal to an axis is the owning factor's face, with its neighbour and
+// boundary (productmesh.hpp:104-134); cells are products of the factors'
+// cells, so sizes, spacings, origins and periodicity concatenate.
+template <int D0>
+CartesianMesh<D0> MakeCartesianProductMesh(const CartesianMesh<D0>& m)
+{
+  return m;
+}
+
+template <int D0, int D1, int... Ds>
+auto MakeCartesianProductMesh(const CartesianMesh<D0>& m0, const CartesianMesh<D1>& m1,
+                              const CartesianMesh<Ds>&... rest)
+{
+  static_assert(D0 + D1 + (0 + ... + Ds) <= GD_MAX_DIM, "at most GD_MAX_DIM axes");
+  return MakeCartesianProductMesh(detail::concat(m0, m1), rest...);
+}
+
 struct FiniteElement {
   int order;
   int basis;

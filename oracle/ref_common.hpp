@@ -210,10 +210,89 @@ int finish( const Args & a, const Op & op, Integer ndofs, Integer nelem )
    return 0;
 }
 
+// GenDiL's own BiCGSTAB_no_alloc (bicgstab.hpp:44) on K = M + dt A (unit mass
+// density, advection field `coef` with boundary field `bc`), b = FillRandom(seed),
+// x0 = 0, tol 1e-10, at most 500 iterations.  Writes n, b, x, iterations,
+// success and the returned relative residual.
+template < int D, typename Space, typename Rule >
+int solve_bicgstab( const Args & a, const Space & fes, const Rule & ir, Integer ndofs )
+{
+   const std::string coef = a.coef, bc = a.bc;
+   auto sigma = []( const std::array< Real, D > & ) -> Real { return 1.0; };
+   auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+   auto g = [bc]( const std::array< Real, D > & X ) -> Real { return boundary_value< D >( bc, X ); };
+   auto M = MakeMassFiniteElementOperator< SerialKernelConfiguration >( fes, ir, sigma );
+   auto A = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv, g );
+   const Real dt = 0.05;
+   Vector tmp( ndofs );
+   auto K = [&]( const Vector & x, Vector & y )
+   {
+      M( x, y );
+      A( x, tmp );
+      Axpy( dt, tmp, y );
+   };
+   auto dot = []( const Vector & u, const Vector & v ) { return Dot( u, v ); };
+   Vector b( ndofs ), x( ndofs ), r( ndofs ), r_hat( ndofs ), p( ndofs ), v( ndofs ), s( ndofs ), t( ndofs );
+   FillRandom( b, a.seed );
+   x = 0.0;
+   auto [ok, iters, relres] = BiCGSTAB_no_alloc( K, b, dot, 500, 1e-10, x, r, r_hat, p, v, s, t );
+   FILE * f = std::fopen( a.out.c_str(), "wb" );
+   if ( !f ) { std::perror( "fopen" ); return 3; }
+   const long long n = ndofs;
+   std::fwrite( &n, sizeof( n ), 1, f );
+   std::fwrite( b.ReadHostData(), sizeof( Real ), ndofs, f );
+   std::fwrite( x.ReadHostData(), sizeof( Real ), ndofs, f );
+   const double tail[3] = { (double)iters, ok ? 1.0 : 0.0, (double)relres };
+   std::fwrite( tail, sizeof( double ), 3, f );
+   std::fclose( f );
+   return 0;
+}
+
+template < int D, int P, int Q, bool Lobatto >
+int run( const Args & a )
+{
+   auto fes = make_space< D, P, Q, Lobatto >( a, std::make_index_sequence< D >{} );
+   auto ir = make_rule< Q >( std::make_index_sequence< D >{} );
+   const Integer ndofs = fes.GetNumberOfFiniteElementDofs();
+   const Integer nelem = fes.GetNumberOfFiniteElements();
+   const std::string coef = a.coef;
+   const std::string bc = a.bc;
+   if ( a.mode == "bicgstab" ) return solve_bicgstab< D >( a, fes, ir, ndofs );
+   if ( a.op == "mass" )
+   {
+      auto sigma = [coef]( const std::array< Real, D > & X ) -> Real { return mass_density< D >( coef, X ); };
+      auto op = MakeMassFiniteElementOperator< SerialKernelConfiguration >( fes, ir, sigma );
+      return finish( a, op, ndofs, nelem );
+   }
+   if ( a.op == "adv" )
+   {
+      auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+      if ( bc == "none" )
+      {
+         auto op = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv );
+         return finish( a, op, ndofs, nelem );
+      }
+      auto g = [bc]( const std::array< Real, D > & X ) -> Real { return boundary_value< D >( bc, X ); };
+      auto op = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv, g );
+      return finish( a, op, ndofs, nelem );
+   }
+   std::fprintf( stderr, "unknown op %s\n", a.op.c_str() );
+   return 2;
+}
+
 using Runner = int ( * )( const Args & );
+
+struct CaseEntry { int D, P, Q; char basis; Runner fn; };
+
+#define GD_CASE( D, P, Q ) { D, P, Q, 'L', &::gdref::run< D, P, Q, true > }
+#define GD_CASE_G( D, P, Q ) { D, P, Q, 'G', &::gdref::run< D, P, Q, false > }
 
 // Entry points of the other translation units of the harness; each returns 2
 // when the requested case is not instantiated.
 int run_padapt_case( const Args & a );
+// mass / adv / bicgstab cases of 5D meshes (ref_highdim.cpp) and of the (n, q)
+// pairs of GenDiL's range benchmarks in 3D, 4D and 6D (ref_rangebench.cpp)
+int run_highdim_case( const Args & a );
+int run_rangebench_case( const Args & a );
 
 } // namespace gdref

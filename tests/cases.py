@@ -144,6 +144,19 @@ CASES = [
     # GenDiL's 6D benchmark operator: 3D x 2D x 1D product mesh (h 1, 1.2345, 1),
     # Legendre p=1, 3 points, beta = 1, no boundary data
     Case("adv6dprod_ones_p1q3_gl", "adv6dprod", 6, 1, 3, "G", 2, False, "ones", "none"),
+    # 5D, among them the (order, points) pairs of GenDiL's 5D range benchmarks
+    # (Legendre, beta = 1, boundary faces skipped; advection q = p + 1, mass q = p + 2)
+    Case("adv5d_ones_gl_p1q2", "adv", 5, 1, 2, "G", 2, False, "ones", "none"),
+    Case("adv5d_trig_p2q3_bcpoly", "adv", 5, 2, 3, "L", 2, False, "trig", "poly"),
+    Case("adv5d_const_p3q4_per", "adv", 5, 3, 4, "L", 2, True, "const", "none"),
+    Case("mass5d_gl_p2q4_poly", "mass", 5, 2, 4, "G", 2, False, "poly", "none"),
+    # the highest (order, points) pairs of GenDiL's 3D, 4D and 6D range benchmarks
+    Case("adv3d_ones_gl_p5q7", "adv", 3, 5, 7, "G", 2, False, "ones", "none"),
+    Case("mass3d_gl_p6q8_poly", "mass", 3, 6, 8, "G", 2, False, "poly", "none"),
+    Case("adv4d_ones_gl_p4q5", "adv", 4, 4, 5, "G", 2, False, "ones", "none"),
+    Case("mass4d_gl_p4q6_poly", "mass", 4, 4, 6, "G", 2, False, "poly", "none"),
+    Case("adv6d_ones_gl_p2q3", "adv", 6, 2, 3, "G", 2, False, "ones", "none"),
+    Case("mass6d_gl_p2q4_one", "mass", 6, 2, 4, "G", 2, False, "one", "none"),
 ]
 
 BY_NAME = {c.name: c for c in CASES}

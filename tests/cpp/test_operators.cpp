@@ -1,8 +1,10 @@
 // C++20 host API test: the header-only operators (include/gendil_amd/
 // operators.hpp) against the oracle (liboracle.so, test infrastructure), and
 // the Krylov solvers.  Exit status 0 = every check passed.
+#include <array>
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "gd_oracle.h"
@@ -220,6 +222,32 @@ int main()
                mixed ? "GMRES mixed (M + dt A) 4^3: oracle residual" : "GMRES FP64 (M + dt A) 4^3: oracle residual",
                r);
       }
+    }
+    // ---- product meshes: GenDiL's 6D benchmark mesh (benchmark-advection-6d.cpp:21-38) ----
+    {
+      const auto pm = ga::MakeCartesianProductMesh(ga::Cartesian3DMesh(1.0, 2, 3, 4), ga::Cartesian2DMesh(1.2345, 5, 6),
+                                                   ga::Cartesian1DMesh(0.5, 7, -1.0, true));
+      static_assert(std::is_same_v<std::remove_const_t<decltype(pm)>, ga::CartesianMesh<6>>);
+      const bool ok = pm.sizes == std::array<int64_t, 6>{2, 3, 4, 5, 6, 7} && pm.h[2] == 1.0 && pm.h[3] == 1.2345 &&
+                      pm.h[5] == 0.5 && pm.origin[5] == -1.0 && pm.periodic[5] && !pm.periodic[4] &&
+                      pm.num_cells() == 2 * 3 * 4 * 5 * 6 * 7;
+      expect(ok, "MakeCartesianProductMesh(3D, 2D, 1D) = Cartesian mesh of the axes", ok ? 0.0 : 1.0);
+      // the operator on the product mesh is the operator on that Cartesian mesh
+      const auto p6 = ga::MakeCartesianProductMesh(ga::Cartesian3DMesh(1.0, 2, 2, 2), ga::Cartesian2DMesh(1.2345, 2, 2),
+                                                   ga::Cartesian1DMesh(1.0, 2));
+      ga::CartesianMesh<6> c6{{2, 2, 2, 2, 2, 2}, {1.0, 1.0, 1.0, 1.2345, 1.2345, 1.0}, {}, {}};
+      ga::VectorField<6> ones;
+      for (auto& comp : ones.components) comp = ga::SeparableField::Constant(1.0);
+      auto fe = ga::MakeLegendreFiniteElement(1);
+      auto A1 = ga::MakeAdvectionOperator(ga::MakeFiniteElementSpace(p6, fe), ga::MakeIntegrationRule(3), ones);
+      auto A2 = ga::MakeAdvectionOperator(ga::MakeFiniteElementSpace(c6, fe), ga::MakeIntegrationRule(3), ones);
+      const int64_t n = A1.num_dofs();
+      ga::Vector<double> x(n), y1(n), y2(n);
+      x.FromHost(ga::FillRandomHost(n, 12345));
+      A1(x, y1);
+      A2(x, y2);
+      const double e = rel_l2(y1.ToHost(), y2.ToHost());
+      expect(e == 0.0, "advection on the product mesh = on the Cartesian mesh", e);
     }
     // ---- errors cross the ABI as exceptions ----
     {
