@@ -18,7 +18,10 @@ bicgstab pinned by GenDiL's own BiCGSTAB_no_alloc run through _ref/gendil_ref
 (tests/golden/bicgstab3d_c2_p1q2_bc0.npz) and by dense solves; the face-centric,
 intermesh and h-refined face operators (advection_faces, hmesh_operator) pinned
 by GenDiL's own face operators through _ref/gendil_ref (tests/golden/face*,
-imesh*, hmesh*), conservation and the split-mesh identity.
+imesh*, hmesh*), conservation and the split-mesh identity; the SIPDG
+diffusion apply pinned by GenDiL's own DiffusionOperator through
+_ref/gendil_ref (tests/golden/diff*), symmetry for sigma = -1 and the constant
+null space on periodic meshes.
 """
 from __future__ import annotations
 
@@ -94,6 +97,7 @@ def lib():
         h.gdo_points.argtypes = [C.c_int32, C.c_int32, pd, pd]
         h.gdo_axis_points.argtypes = [P, C.c_int32, pd]
         h.gdo_num_dofs.argtypes = [P]
+        h.gdo_diffusion_apply.argtypes = [P, C.POINTER(GdoSeparable), C.c_double, C.c_double, pd, pd]
         h.gdo_num_dofs.restype = C.c_int64
         _lib = h
     return _lib
@@ -225,6 +229,21 @@ class Problem:
         if rc:
             raise RuntimeError(f"gdo_apply_transpose failed ({rc})")
         return z
+
+    def diffusion(self, x: np.ndarray, mu, sigma: float, kappa: float) -> np.ndarray:
+        """SIPDG diffusion apply (gdo_diffusion_apply) on this problem's mesh,
+        basis and rule, diffusivity `mu` (a separable field or a constant)."""
+        from gendil_amd import fields as F
+
+        if isinstance(mu, (int, float)):
+            mu = F.SeparableField.constant(float(mu))
+        m = self._sep(mu)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.num_dofs)
+        rc = lib().gdo_diffusion_apply(C.byref(self.p), C.byref(m), sigma, kappa, _dp(x), _dp(y))
+        if rc:
+            raise RuntimeError(f"gdo_diffusion_apply failed ({rc})")
+        return y
 
     def grad_beta(self, u, r, face_counts):
         m = self.fes.mesh

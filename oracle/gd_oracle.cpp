@@ -150,7 +150,7 @@ double lagrange_deriv(const std::vector<double>& xs, int k, double x)
 // replaces the normal dimension by ZeroPoint / OnePoint with weight 1).
 struct Tables {
   int n = 0, q = 0;
-  std::vector<double> xq, wq, B, Gq, b0, b1;
+  std::vector<double> xq, wq, B, Gq, b0, b1, d0, d1;  // d0, d1: l_i'(0), l_i'(1)
 };
 
 Tables make_tables(int basis, int n, int q)
@@ -172,9 +172,13 @@ Tables make_tables(int basis, int n, int q)
     for (int i = 0; i < n; ++i) t.B[a * n + i] = lagrange_value(xn, i, t.xq[a]);
   for (int a = 0; a < q; ++a)
     for (int b = 0; b < q; ++b) t.Gq[a * q + b] = lagrange_deriv(t.xq, b, t.xq[a]);
+  t.d0.resize(n);
+  t.d1.resize(n);
   for (int i = 0; i < n; ++i) {
     t.b0[i] = lagrange_value(xn, i, 0.0);
     t.b1[i] = lagrange_value(xn, i, 1.0);
+    t.d0[i] = lagrange_deriv(xn, i, 0.0);
+    t.d1[i] = lagrange_deriv(xn, i, 1.0);
   }
   return t;
 }
@@ -448,14 +452,16 @@ void grad_cell(const Tables& T, int D, int d, const double* in, double* out)
 // the face rule of nqf points in the tangential dims (face tables of
 // doftoquad.hpp with the normal dimension replaced by Zero/OnePoint).
 // Result extent: nqf in d != a, 1 in d == a.
-void trace(const Tables& Tt, int D, int a, bool at_one, const double* u, double* out, double* tmp)
+// `row` is the normal dimension's 1-point table: the basis values (b0 / b1)
+// for the trace, their derivatives (d0 / d1) for the normal derivative.
+void trace_row(const Tables& Tt, int D, int a, const double* row, const double* u, double* out, double* tmp)
 {
   Shape s{D, {}};
   for (int d = 0; d < D; ++d) s.ext[d] = Tt.n;
   std::vector<double> cur(u, u + s.size());
   for (int d = D - 1; d >= 0; --d) {
     Shape so;
-    if (d == a) contract(cur.data(), s, d, at_one ? Tt.b1.data() : Tt.b0.data(), Tt.n, 1, false, tmp, so);
+    if (d == a) contract(cur.data(), s, d, row, Tt.n, 1, false, tmp, so);
     else contract(cur.data(), s, d, Tt.B.data(), Tt.n, Tt.q, false, tmp, so);
     cur.assign(tmp, tmp + so.size());
     s = so;
@@ -463,21 +469,32 @@ void trace(const Tables& Tt, int D, int a, bool at_one, const double* u, double*
   std::memcpy(out, cur.data(), cur.size() * sizeof(double));
 }
 
-// Transpose of trace: add face values (extent 1 in dim a) into n^D DoFs
+void trace(const Tables& Tt, int D, int a, bool at_one, const double* u, double* out, double* tmp)
+{
+  trace_row(Tt, D, a, at_one ? Tt.b1.data() : Tt.b0.data(), u, out, tmp);
+}
+
+// Transpose of trace_row: add face values (extent 1 in dim a) into n^D DoFs
 // (ApplyAddTestFunctions for faces, applytestfunctions.hpp:251).
-void trace_transpose_add(const Tables& Tt, int D, int a, bool at_one, const double* in, double* out, double* tmp)
+void trace_row_transpose_add(const Tables& Tt, int D, int a, const double* row, const double* in, double* out,
+                             double* tmp)
 {
   Shape s{D, {}};
   for (int d = 0; d < D; ++d) s.ext[d] = (d == a) ? 1 : Tt.q;
   std::vector<double> cur(in, in + s.size());
   for (int d = 0; d < D; ++d) {
     Shape so;
-    if (d == a) contract(cur.data(), s, d, at_one ? Tt.b1.data() : Tt.b0.data(), 1, Tt.n, true, tmp, so);
+    if (d == a) contract(cur.data(), s, d, row, 1, Tt.n, true, tmp, so);
     else contract(cur.data(), s, d, Tt.B.data(), Tt.n, Tt.n, true, tmp, so);
     cur.assign(tmp, tmp + so.size());
     s = so;
   }
   for (size_t i = 0; i < cur.size(); ++i) out[i] += cur[i];
+}
+
+void trace_transpose_add(const Tables& Tt, int D, int a, bool at_one, const double* in, double* out, double* tmp)
+{
+  trace_row_transpose_add(Tt, D, a, at_one ? Tt.b1.data() : Tt.b0.data(), in, out, tmp);
 }
 
 // Face quadrature point loop helper: iterates the tangential multi-index
@@ -719,6 +736,112 @@ void element_apply(const Problem& P, Mode mode, int64_t e, const double* x, doub
   for (int64_t i = 0; i < nd; ++i) ye[i] = out[i] + p->adv_coef * adv[i];
 }
 
+// One element of the SIPDG diffusion apply, y overwritten: the volume term of
+// DiffusionElementOperator (diffusionoperator.hpp:44-101) plus the face terms
+// of DiffusionFaceOperator (:130-331), interior faces (:170-272) and boundary
+// faces (:273-330).  Gradients at the cell rule points are the collocated
+// derivative of the interpolated values (InterpolateGradientSerial,
+// interpolategradientserial.hpp:107-123, branch num_quads >= num_dofs); on a
+// face the normal derivative contracts the normal dimension with l_i'(0) or
+// l_i'(1) (InterpolateGradientSumFactSerial, :39-48, num_quads = 1).
+void diffusion_element(const Problem& P, const gdo_separable& mu, double sig, double kap, int64_t e, const double* x,
+                       double* y, Work& W)
+{
+  const gdo_problem* p = P.p;
+  const int D = P.D;
+  const int n = p->order + 1;
+  const int q = p->nq;
+  const Tables& T = cache().get(p->basis, n, q);
+  const int64_t nd = ipow(n, D);
+  const int64_t nqd = ipow(q, D);
+  int64_t sub[kMaxDim];
+  unravel(e, P.M, D, sub);
+  const double* ue = x + P.dof0(e);
+  double* tmp = W.t2.data();
+  std::vector<double> out(nd, 0.0);
+
+  // ---- volume: B^T sum_d Gq_d^T (w detJ mu / h_d^2) Gq_d B u ----
+  double* Bu = W.t0.data();
+  interp_cell(T, D, ue, Bu, tmp);
+  double* gu[kMaxDim];
+  for (int d = 0; d < D; ++d) {
+    gu[d] = W.dbu[d].data();
+    grad_cell(T, D, d, Bu, gu[d]);
+  }
+  int k[kMaxDim];
+  for (int64_t i = 0; i < nqd; ++i) {
+    int64_t t = i;
+    for (int d = 0; d < D; ++d) {
+      k[d] = (int)(t % q);
+      t /= q;
+    }
+    const double w = cell_weight(T, k, D) * P.detJ * eval_sep(mu, sub, k, D, q);
+    for (int d = 0; d < D; ++d) {
+      double g = gu[d][i] * P.invh[d];  // ApplyMapping (physical gradient)
+      g *= P.invh[d];                   // ApplyMappingTranspose
+      gu[d][i] = w * g;
+    }
+  }
+  std::vector<double> gt(nqd);
+  grad_transpose_cell(T, D, gu, gt.data(), tmp);
+  test_cell(T, D, gt.data(), out.data(), tmp);
+
+  // ---- faces (FaceLoop with an interior and a boundary lambda) ----
+  for (int f = 0; f < 2 * D; ++f) {
+    const int a = f % D;
+    const int sign = f < D ? -1 : 1;
+    const int64_t nb = P.neighbor(sub, f);
+    const bool boundary = nb < 0;
+    const FaceIter it(D, a, q);
+    const double* bo = sign > 0 ? T.b1.data() : T.b0.data();
+    const double* go = sign > 0 ? T.d1.data() : T.d0.data();
+    std::vector<double> tv(it.count), tg(it.count), nv(it.count, 0.0), ng(it.count, 0.0);
+    std::vector<double> du(it.count), dg(it.count);
+    trace_row(T, D, a, bo, ue, tv.data(), tmp);
+    trace_row(T, D, a, go, ue, tg.data(), tmp);
+    if (!boundary) {
+      // the neighbour's face is its opposite one (xi_a = 0 when this face is at 1)
+      const double* unb = x + P.dof0(nb);
+      const double* bn = sign > 0 ? T.b0.data() : T.b1.data();
+      const double* gn = (sign > 0) != (p->plant == 4) ? T.d0.data() : T.d1.data();
+      trace_row(T, D, a, bn, unb, nv.data(), tmp);
+      trace_row(T, D, a, gn, unb, ng.data(), tmp);
+    }
+    int kf[kMaxDim];
+    for (int64_t i = 0; i < it.count; ++i) {
+      it.sub(i, kf);
+      const double w = face_weight(T, kf, D, a);
+      for (int d = 0; d < D; ++d) k[d] = kf[d];
+      k[a] = sign < 0 ? q : q + 1;  // face point seen from this element
+      const double m = eval_sep(mu, sub, k, D, q);
+      const double g_own = tg[i] * P.invh[a];  // normal component of inv_J^T grad
+      double avg, jump, half;
+      if (!boundary) {
+        const double g_nb = ng[i] * P.invh[a];
+        avg = 0.5 * m * (g_own + g_nb);
+        jump = tv[i] - nv[i];
+        half = p->plant == 3 ? 1.0 : 0.5;
+      } else {
+        avg = m * g_own;
+        jump = tv[i];
+        half = 1.0;
+      }
+      const double nrm = (double)sign * P.invh[a];  // computephysicalnormal.hpp:27-35
+      const double dot = avg * nrm;
+      const double norm = std::sqrt(nrm * nrm);
+      const double h_inv = norm;
+      du[i] = -w * P.detJ * dot + w * P.detJ * norm * kap * h_inv * m * jump;
+      const double ws = sig * w * P.detJ * jump * m * half;
+      dg[i] = (ws * nrm) * P.invh[a];  // ApplyMappingTranspose of the normal
+    }
+    // ApplyValuesAndGradientTestFunctions<true> on this side
+    trace_row_transpose_add(T, D, a, bo, du.data(), out.data(), tmp);
+    trace_row_transpose_add(T, D, a, go, dg.data(), out.data(), tmp);
+  }
+  double* ye = y + P.dof0(e);
+  for (int64_t i = 0; i < nd; ++i) ye[i] = out[i];
+}
+
 int64_t num_dofs(const Problem& P)
 {
   if (P.padapt) {
@@ -898,6 +1021,15 @@ int gdo_grad_beta(const gdo_problem* p, const double* u, const double* r, double
       }
     }
   });
+  return 0;
+}
+
+int gdo_diffusion_apply(const gdo_problem* p, const gdo_separable* mu, double sigma, double kappa, const double* x,
+                        double* y)
+{
+  if (p->elem_order) return 1;  // uniform order only
+  Problem P(p);
+  parallel_elements(P, [&](int64_t e, Work& W) { diffusion_element(P, *mu, sigma, kappa, e, x, y, W); });
   return 0;
 }
 

@@ -72,7 +72,8 @@ typedef struct gdo_problem {
    * with the smaller of the two cell rules instead of the larger; 2 = the
    * neighbour's trace on such a face taken from its node slab with two
    * tangential node axes exchanged (D >= 3).  The first is the same on both
-   * sides of a face, so conservation cannot see it. */
+   * sides of a face, so conservation cannot see it.  3 and 4: see
+   * gdo_diffusion_apply. */
   int32_t plant;
 } gdo_problem;
 
@@ -94,6 +95,15 @@ int gdo_apply_transpose(const gdo_problem* p, const double* r, double* z);
  * for L = <r, A(beta) u> (adv_coef included). */
 int gdo_grad_beta(const gdo_problem* p, const double* u, const double* r,
                   double* dbeta_cell, double* const* dbeta_face);
+/* SIPDG diffusion (DiffusionOperator, diffusionoperator.hpp:44-331), y
+ * overwritten: diffusivity mu, symmetry parameter sigma (-1 symmetric, 0
+ * incomplete, 1 non-symmetric interior penalty), penalty kappa; faces on a
+ * non-periodic boundary carry the weak homogeneous Dirichlet terms.  Uses the
+ * mesh, basis and rule of p (uniform order; mass/advection fields ignored).
+ * p->plant 3: interior symmetry term without its factor 1/2; 4: the
+ * neighbour's normal derivative taken at the wrong end of its element. */
+int gdo_diffusion_apply(const gdo_problem* p, const gdo_separable* mu, double sigma, double kappa,
+                        const double* x, double* y);
 
 #ifdef __cplusplus
 }

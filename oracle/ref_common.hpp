@@ -294,5 +294,7 @@ int run_padapt_case( const Args & a );
 // pairs of GenDiL's range benchmarks in 3D, 4D and 6D (ref_rangebench.cpp)
 int run_highdim_case( const Args & a );
 int run_rangebench_case( const Args & a );
+// SIPDG diffusion (ref_diffusion.cpp)
+int run_diffusion_case( const Args & a );
 
 } // namespace gdref

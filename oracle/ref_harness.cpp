@@ -19,6 +19,8 @@
 //           own element-centric operator on q = p+1 points, coupled by the
 //           intermesh face operator on face F with q = max(P, PP) + 1; see
 //           ref_padapt.cpp)
+//           | diff (SIPDG diffusion, ref_diffusion.cpp; coef = diffusivity
+//           one | poly | trig, bc = s<sigma>_k<kappa>)
 //   basis : L (Gauss-Lobatto nodes) | G (Gauss-Legendre nodes)
 //   coef  : mass -> one | poly ;  adv -> c2 | bubble | const | trig | phase | yzx | ones
 //   bc    : none (boundary faces skipped) | zero | poly
@@ -202,6 +204,7 @@ int main( int argc, char ** argv )
    else a.pairs = std::atoi( argv[12] );
    if ( a.op == "adv6dprod" ) return run_product( a );
    if ( a.op.rfind( "padapt_", 0 ) == 0 ) return run_padapt_case( a );
+   if ( a.op == "diff" ) return run_diffusion_case( a );
    if ( a.op == "advface" )
    {
       for ( const CaseEntry & e : kFaceEntries )

@@ -396,3 +396,89 @@ PADAPT_CASES = [
 ]
 
 PADAPT_BY_NAME = {c.name: c for c in PADAPT_CASES}
+
+
+def diffusivity_field(kind: str, D: int):
+    """The diffusivity fields of oracle/ref_diffusion.cpp as separable fields."""
+    if kind == "one":
+        return SeparableField.constant(1.0)
+    if kind == "poly":  # 1 + x_0 x_{D-1} + 0.5 x_0^2
+        if D == 1:
+            return SeparableField([(1.0, {}), (1.5, {0: lambda s: s * s})])
+        return SeparableField(
+            [(1.0, {}), (1.0, {0: lambda s: s, D - 1: lambda s: s}), (0.5, {0: lambda s: s * s})]
+        )
+    if kind == "trig":
+        if D == 3:
+            # 2 + 0.25 sin(2 pi x) + 0.17 cos(2 pi (2y + 0.13)) + 0.11 sin(2 pi (3z + 0.37))
+            #   + 0.07 sin(A + B + C), A = 2 pi x, B = 4 pi y, C = 6 pi z, expanded
+            sA, cA = (lambda s: np.sin(TWO_PI * s)), (lambda s: np.cos(TWO_PI * s))
+            sB, cB = (lambda s: np.sin(2 * TWO_PI * s)), (lambda s: np.cos(2 * TWO_PI * s))
+            sC, cC = (lambda s: np.sin(3 * TWO_PI * s)), (lambda s: np.cos(3 * TWO_PI * s))
+            return SeparableField([
+                (2.0, {}),
+                (0.25, {0: sA}),
+                (0.17, {1: lambda s: np.cos(TWO_PI * (2.0 * s + 0.13))}),
+                (0.11, {2: lambda s: np.sin(TWO_PI * (3.0 * s + 0.37))}),
+                (0.07, {0: sA, 1: cB, 2: cC}),
+                (0.07, {0: cA, 1: sB, 2: cC}),
+                (0.07, {0: cA, 1: cB, 2: sC}),
+                (-0.07, {0: sA, 1: sB, 2: sC}),
+            ])
+        last = lambda s: np.cos(TWO_PI * (2.0 * s + 0.13))  # noqa: E731
+        if D == 1:
+            return SeparableField([(2.0, {}), (0.25, {0: lambda s: np.sin(TWO_PI * s)}), (0.17, {0: last})])
+        return SeparableField([(2.0, {}), (0.25, {0: lambda s: np.sin(TWO_PI * s)}), (0.17, {D - 1: last})])
+    raise ValueError(kind)
+
+
+@dataclass(frozen=True)
+class DiffCase:
+    """One SIPDG diffusion fixture: GenDiL's DiffusionOperator on an n^D mesh
+    (oracle/ref_diffusion.cpp), sigma the symmetry parameter, kappa the penalty."""
+    name: str
+    D: int
+    P: int
+    Q: int
+    basis: str
+    n: int
+    periodic: bool
+    coef: str
+    sigma: float
+    kappa: float
+    seed: int = 12345
+
+    @property
+    def harness_bc(self):
+        return f"s{self.sigma:g}_k{self.kappa:g}"
+
+    def space(self):
+        mesh = G.CartesianMesh.unit(self.D, self.n, self.periodic)
+        fe = G.MakeLobattoFiniteElement(self.P) if self.basis == "L" else G.MakeLegendreFiniteElement(self.P)
+        return G.MakeFiniteElementSpace(mesh, fe), G.MakeIntegrationRule(self.Q)
+
+    def mu(self):
+        return diffusivity_field(self.coef, self.D)
+
+
+DIFF_CASES = [
+    # test-diffusion-3d-generic.cpp:78-167: n = 2 periodic, GLL, q = p + 3,
+    # sigma = -1, kappa = (p + 1)^2
+    DiffCase("diff3d_trig_p1q4_per", 3, 1, 4, "L", 2, True, "trig", -1.0, 4.0),
+    DiffCase("diff3d_trig_p2q5_per", 3, 2, 5, "L", 2, True, "trig", -1.0, 9.0),
+    DiffCase("diff3d_trig_p3q6_per", 3, 3, 6, "L", 2, True, "trig", -1.0, 16.0),
+    # boundary faces (weak homogeneous Dirichlet), the other two symmetry parameters
+    DiffCase("diff3d_trig_p3q6_bnd", 3, 3, 6, "L", 3, False, "trig", -1.0, 16.0),
+    DiffCase("diff3d_poly_p3q4_nip", 3, 3, 4, "L", 3, False, "poly", 1.0, 10.0),
+    DiffCase("diff3d_one_p3q5_iip", 3, 3, 5, "L", 4, True, "one", 0.0, 16.0),
+    DiffCase("diff3d_gl_poly_p2q4", 3, 2, 4, "G", 3, False, "poly", -1.0, 9.0),
+    # test-diffusion-symmetry-1d-minimal.cpp:10-40 (sigma = 1, kappa = 0)
+    DiffCase("diff1d_one_p1q4_per", 1, 1, 4, "L", 2, True, "one", 1.0, 0.0),
+    # poisson-convergence-{1d,2d,6d}.cpp: Legendre, q = p + 2, sigma = -1, kappa = (p + 1)^2
+    DiffCase("diff1d_gl_poly_p3q5", 1, 3, 5, "G", 5, False, "poly", -1.0, 16.0),
+    DiffCase("diff2d_gl_trig_p2q4", 2, 2, 4, "G", 4, False, "trig", -1.0, 9.0),
+    DiffCase("diff2d_trig_p3q4_per", 2, 3, 4, "L", 4, True, "trig", -1.0, 16.0),
+    DiffCase("diff6d_gl_one_p1q3", 6, 1, 3, "G", 2, False, "one", -1.0, 4.0),
+]
+
+DIFF_BY_NAME = {c.name: c for c in DIFF_CASES}

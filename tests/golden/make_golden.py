@@ -27,7 +27,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 import oracle  # noqa: E402
-from cases import CASES, FACE_CASES, PADAPT_CASES  # noqa: E402
+from cases import CASES, DIFF_CASES, FACE_CASES, PADAPT_CASES  # noqa: E402
 
 
 def main(only=None):
@@ -62,6 +62,16 @@ def main(only=None):
                 continue
             path = Path(tmp) / f"{c.name}.bin"
             x, y = oracle.ref_fixture(c.harness_op, c.D, c.PM, 0, "L", c.n, c.periodic, c.coef, "none", c.seed, path)
+            digest = hashlib.sha256(x.tobytes()).hexdigest()
+            np.savez_compressed(HERE / f"{c.name}.npz", y=y, x_sha256=np.array(digest), n=np.array(x.size))
+            print(f"{c.name}: {x.size} dofs, |y|={np.linalg.norm(y):.6e}")
+        # GenDiL's SIPDG DiffusionOperator (ref_diffusion.cpp)
+        for c in DIFF_CASES:
+            if not wanted(c.name):
+                continue
+            path = Path(tmp) / f"{c.name}.bin"
+            x, y = oracle.ref_fixture("diff", c.D, c.P, c.Q, c.basis, c.n, int(c.periodic), c.coef, c.harness_bc,
+                                      c.seed, path)
             digest = hashlib.sha256(x.tobytes()).hexdigest()
             np.savez_compressed(HERE / f"{c.name}.npz", y=y, x_sha256=np.array(digest), n=np.array(x.size))
             print(f"{c.name}: {x.size} dofs, |y|={np.linalg.norm(y):.6e}")
