@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-padapt", action="store_true",
                     help="skip the 6D p-adaptive, product-mesh, adjoint and face-operator entries")
     ap.add_argument("--no-mass", action="store_true", help="skip the 32^3 mass apply")
+    ap.add_argument("--no-diffusion", action="store_true", help="skip the 64^3 SIPDG diffusion apply")
     ap.add_argument("--face-ops-only", action="store_true",
                     help="only time the face-centric operators (one JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -208,6 +209,53 @@ def mass_32(steps, warmup, n=32, pairs=10):
                                       "OMP_PLACES": "cores"})
             out["cpu_reference"] = {"value": r["dofs_per_s"] / 1e9, "unit": "GDoF/s", "cores": threads,
                                     "kind": "reference", "sample": "1 warm-up + 5 timed apply pairs"}
+    except Exception as e:  # the GPU figure stands on its own
+        out["cpu_reference"] = {"error": str(e)[:200]}
+    return out
+
+
+def diffusion_64(steps, warmup, n=64, q=5):
+    """The SIPDG diffusion apply (GenDiL's DiffusionOperator) on the 64^3
+    periodic mesh, GLL p=3, q = p + 2 Gauss points (its Poisson examples'
+    rule), diffusivity 2 + 0.25 sin 2 pi x + 0.17 cos 2 pi (2y + 0.13) +
+    0.11 sin 2 pi (3z + 0.37) + 0.07 sin 2 pi (x + 2y + 3z), sigma = -1,
+    kappa = 16; FP64, three rotating vector pairs (402 MB each way).
+    Algorithmic bytes 16 per DoF.  Beside it GenDiL's own OpenMP apply on the
+    32^3 mesh where it is built."""
+    import torch
+
+    import gendil_amd as G
+    from gendil_amd import workloads as W
+
+    fes = G.MakeFiniteElementSpace(G.CartesianMesh.unit(3, n, True), G.MakeLobattoFiniteElement(3))
+    op = G.MakeDiffusionOperator(fes, G.MakeIntegrationRule(q), W.diffusivity_trig_3d(), -1.0, 16.0)
+    N = op.num_dofs
+    x0 = torch.as_tensor(G.fill_random(N, 12345), device="cuda")
+    xs = [x0, x0 * 0.5, x0 * -0.25]
+    ys = [torch.empty_like(x0) for _ in range(3)]
+    ms = time_applies(op, xs, ys, steps, warmup)
+    t = ms * 1e-3
+    out = {
+        "workload": f"SIPDG diffusion apply, 3D {n}^3 periodic, GLL p=3, {q} GL pts/dim, trig diffusivity, "
+                    "sigma = -1, kappa = 16, FP64",
+        "dofs": N,
+        "value": N / t / 1e9,
+        "unit": "GDoF/s",
+        "ms_per_step": ms,
+        "roofline": {"bound": "hbm", "achieved": 16.0 * N / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": 16.0 * N / t / 1e9 / HBM_PEAK_GBS},
+    }
+    del xs, ys
+    try:
+        import oracle
+
+        if oracle.ref_available():
+            threads = host_cpu()["threads"]
+            r = oracle.ref_bench("diff", 3, 3, q, "L", 32, True, "trig", "s-1_k16", 12345, 2,
+                                 env={"OMP_NUM_THREADS": str(threads), "OMP_PROC_BIND": "close",
+                                      "OMP_PLACES": "cores"})
+            out["cpu_reference"] = {"value": r["dofs_per_s"] / 1e9, "unit": "GDoF/s", "cores": threads,
+                                    "kind": "reference", "sample": "32^3 mesh, 1 warm-up + 2 timed apply pairs"}
     except Exception as e:  # the GPU figure stands on its own
         out["cpu_reference"] = {"error": str(e)[:200]}
     return out
@@ -616,6 +664,9 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_mass:
         result["mass32"] = mass_32(args.steps, args.warmup)
+        torch.cuda.empty_cache()
+    if not args.no_diffusion:
+        result["diffusion64"] = diffusion_64(max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()
     if not args.no_padapt:
         result["padapt6d"] = padapt_6d(max(10, args.steps // 4), 3)

@@ -71,6 +71,8 @@ def __getattr__(name):
         "NonconformingCartesianIntermeshFaceConnectivity",
         "MakeCartesianInteriorFaceConnectivity",
         "MakeAdvectionFaceOperator",
+        "DiffusionOperator",
+        "MakeDiffusionOperator",
     ):
         from . import operators
 

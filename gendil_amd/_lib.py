@@ -97,12 +97,15 @@ SIGNATURES = {
     "gd_plan_set_beta_affine": (C.c_int, [_p, _pd, _pd]),
     "gd_plan_set_beta_pointwise_f64": (C.c_int, [_p, _p, _ppd]),
     "gd_plan_set_bc": (C.c_int, [_p, _i32, C.POINTER(GdSeparable)]),
+    "gd_plan_set_diffusion": (C.c_int, [_p, C.POINTER(GdSeparable), C.c_double, C.c_double]),
     "gd_apply_f64": (C.c_int, [_p, _d, _d, _p, _p, _p]),
     "gd_apply_f32": (C.c_int, [_p, _f, _f, _p, _p, _p]),
     "gd_mass_apply_f64": (C.c_int, [_p, _p, _p, _p]),
     "gd_mass_apply_f32": (C.c_int, [_p, _p, _p, _p]),
     "gd_adv_apply_f64": (C.c_int, [_p, _p, _p, _p]),
     "gd_adv_apply_f32": (C.c_int, [_p, _p, _p, _p]),
+    "gd_diff_apply_f64": (C.c_int, [_p, _p, _p, _p]),
+    "gd_diff_apply_f32": (C.c_int, [_p, _p, _p, _p]),
     "gd_apply_T_f64": (C.c_int, [_p, _d, _d, _p, _p, _p]),
     "gd_adv_apply_T_f64": (C.c_int, [_p, _p, _p, _p]),
     "gd_adv_grad_beta_f64": (C.c_int, [_p, _d, _p, _p, _p, _ppd, _p]),

@@ -36,7 +36,8 @@ struct FieldDev {
 
 // 1D tables for one (basis, n, q) pair, laid out contiguously on the device:
 // B[q][n] values, Gq[q][q] collocated derivative, b0[n], b1[n] face traces,
-// w[q] weights, xq[q] points.
+// w[q] weights, xq[q] points, Gb[q][n] basis derivative, d0[n], d1[n] basis
+// derivative at the element ends.
 template <typename T>
 struct Tables1D {
   const T* B;
@@ -46,6 +47,8 @@ struct Tables1D {
   const T* w;
   const T* xq;
   const T* Gb;  // basis derivative at quadrature points Gb[q][i] = l_i'(x_q)
+  const T* d0;  // l_i'(0)
+  const T* d1;  // l_i'(1)
 };
 
 // Kernel parameters (passed by value as kernel arguments).
@@ -74,6 +77,10 @@ struct Params {
   // grad-beta outputs
   T* dbeta_cell;
   T* dbeta_face[kMaxDim];
+  // SIPDG diffusion: diffusivity, symmetry parameter, penalty
+  const FieldDev<T>* mu;
+  T ip_sigma;
+  T ip_kappa;
   // XCD-aware block remap
   int num_blocks;
 };
@@ -84,7 +91,7 @@ struct TableSet {
   void* dev32 = nullptr;  // contiguous device block (float)
   Tables1D<double> t64{};
   Tables1D<float> t32{};
-  std::vector<double> B, Gq, b0, b1, w, xq, Gb;  // host copies
+  std::vector<double> B, Gq, b0, b1, w, xq, Gb, d0, d1;  // host copies
 };
 
 struct HostField {
@@ -124,11 +131,16 @@ struct gd_plan {
   gd::HostField sigma_host;
   gd::HostField bc_host;
   int bc_mode = GD_BC_NONE;
+  // SIPDG diffusion (gd_plan_set_diffusion)
+  bool diff_set = false;
+  gd::HostField mu_host;
+  double ip_sigma = 0.0, ip_kappa = 0.0;
   double affA[gd::kMaxDim][gd::kMaxDim] = {};
   double affc[gd::kMaxDim] = {};
   const double* beta_cell = nullptr;
   const double* beta_face[gd::kMaxDim] = {};
-  // device copies of separable fields: [0..dim) beta, dim sigma, dim+1 bc
+  // device copies of separable fields: [0..dim) beta, dim sigma, dim+1 bc,
+  // dim+2 diffusivity
   void* fields64 = nullptr;
   void* fields32 = nullptr;
   std::vector<gd::FieldDev<double>> fields64_host;  // host copies of the device descriptors

@@ -1,5 +1,6 @@
 // p-adaptive Cartesian DG apply (GLL basis, Gauss rule with q_e = p_e + 1
-// points per dimension, n_e = p_e + 1 in {2, 3, 4}) in one persistent launch.
+// points per dimension, n_e = p_e + 1 in {2, 3, 4}) in one launch, one block
+// per work item.
 //
 // What is computed per element e is the element-centric operator of the
 // uniform kernels with per-element orders:

@@ -158,7 +158,13 @@ int build_tables(int basis, int n, int q, TableSet& ts)
   ts.Gb.resize((size_t)q * n);
   for (int a = 0; a < q; ++a)
     for (int i = 0; i < n; ++i) ts.Gb[a * n + i] = dlag(xn, i, ts.xq[a]);
-  // device upload: B | Gq | b0 | b1 | w | xq | Gb
+  ts.d0.resize(n);
+  ts.d1.resize(n);
+  for (int i = 0; i < n; ++i) {
+    ts.d0[i] = dlag(xn, i, 0.0);
+    ts.d1[i] = dlag(xn, i, 1.0);
+  }
+  // device upload: B | Gq | b0 | b1 | w | xq | Gb | d0 | d1
   std::vector<double> blob;
   blob.insert(blob.end(), ts.B.begin(), ts.B.end());
   blob.insert(blob.end(), ts.Gq.begin(), ts.Gq.end());
@@ -167,6 +173,8 @@ int build_tables(int basis, int n, int q, TableSet& ts)
   blob.insert(blob.end(), ts.w.begin(), ts.w.end());
   blob.insert(blob.end(), ts.xq.begin(), ts.xq.end());
   blob.insert(blob.end(), ts.Gb.begin(), ts.Gb.end());
+  blob.insert(blob.end(), ts.d0.begin(), ts.d0.end());
+  blob.insert(blob.end(), ts.d1.begin(), ts.d1.end());
   std::vector<float> blob32(blob.begin(), blob.end());
   GD_HIP(hipMalloc(&ts.dev64, blob.size() * sizeof(double)));
   GD_HIP(hipMalloc(&ts.dev32, blob32.size() * sizeof(float)));
@@ -187,6 +195,10 @@ int build_tables(int basis, int n, int q, TableSet& ts)
     t.xq = base + o;
     o += q;
     t.Gb = base + o;
+    o += ts.Gb.size();
+    t.d0 = base + o;
+    o += n;
+    t.d1 = base + o;
   };
   fill(static_cast<double*>(ts.dev64), ts.t64);
   fill(static_cast<float*>(ts.dev32), ts.t32);
@@ -236,15 +248,17 @@ void free_fields(gd_plan* p)
   p->fields64 = p->fields32 = nullptr;
 }
 
-// Upload beta[0..dim), sigma, bc as FieldDev arrays (double and float).
+// Upload beta[0..dim), sigma, bc, diffusivity as FieldDev arrays (double and
+// float).
 int upload_fields(gd_plan* p)
 {
   free_fields(p);
-  const int nf = p->dim + 2;
+  const int nf = p->dim + 3;
   std::vector<FieldDev<double>> f64(nf);
   std::vector<FieldDev<float>> f32(nf);
   for (int i = 0; i < nf; ++i) {
-    const HostField& h = i < p->dim ? p->beta_host[i] : (i == p->dim ? p->sigma_host : p->bc_host);
+    const HostField& h = i < p->dim ? p->beta_host[i]
+                                     : (i == p->dim ? p->sigma_host : (i == p->dim + 1 ? p->bc_host : p->mu_host));
     FieldDev<double>& a = f64[i];
     FieldDev<float>& b = f32[i];
     std::memset(&a, 0, sizeof(a));
@@ -334,6 +348,9 @@ Params<T> make_params(const gd_plan* p, const TableSet* ts, T mass_coef, T adv_c
   k.sigma = k.beta + D;
   k.bc = (p->bc_mode == GD_BC_INFLOW && p->bc_host.rank > 0) ? k.beta + D + 1 : nullptr;
   k.bc_mode = p->bc_mode;
+  k.mu = k.beta + D + 2;
+  k.ip_sigma = (T)p->ip_sigma;
+  k.ip_kappa = (T)p->ip_kappa;
   k.beta_kind = p->beta_kind;
   for (int c = 0; c < D; ++c) {
     k.affc[c] = (T)p->affc[c];
@@ -517,6 +534,21 @@ int gd_plan_set_sigma(gd_plan* plan, const gd_separable* sigma)
   int rc = gd::check_sep(plan, sigma, "sigma", false);
   if (rc) return rc;
   gd::copy_sep(plan, sigma, plan->sigma_host, 1.0);
+  return gd::commit_coefficients(plan, true);
+}
+
+int gd_plan_set_diffusion(gd_plan* plan, const gd_separable* mu, double sigma, double kappa)
+{
+  if (!plan) return fail(GD_ERR_INVALID, "gd_plan_set_diffusion: null plan");
+  if (plan->padapt) return fail(GD_ERR_UNSUPPORTED, "gd_plan_set_diffusion: p-adaptive plans have no diffusion apply");
+  if (!std::isfinite(sigma) || !std::isfinite(kappa))
+    return fail(GD_ERR_INVALID, "gd_plan_set_diffusion: sigma and kappa must be finite");
+  int rc = gd::check_sep(plan, mu, "mu", false);
+  if (rc) return rc;
+  gd::copy_sep(plan, mu, plan->mu_host, 1.0);
+  plan->ip_sigma = sigma;
+  plan->ip_kappa = kappa;
+  plan->diff_set = true;
   return gd::commit_coefficients(plan, true);
 }
 

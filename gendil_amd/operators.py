@@ -2,6 +2,7 @@
 
   MakeMassFiniteElementOperator(fes, ir, sigma)       massoperator.hpp:270
   MakeAdvectionOperator(fes, ir, adv[, bc])           advectionoperator.hpp:1416-1528
+  MakeDiffusionOperator(fes, ir, mu, sigma, kappa)    diffusionoperator.hpp:519-537
   op(x, y)  overwrites y (L2 semantics, massoperator.hpp:209-216,
             advectionoperator.hpp:1070-1076)
 
@@ -137,6 +138,17 @@ class Plan:
         self._beta_pw = (cell, list(faces))
         self.beta_kind = L.BETA_POINTWISE
 
+    def set_diffusion(self, mu, sigma: float, kappa: float):
+        """SIPDG diffusivity mu (separable field or constant), symmetry
+        parameter sigma and penalty kappa (gd_plan_set_diffusion)."""
+        if mu is None or isinstance(mu, (int, float)):
+            mu = SeparableField.constant(1.0 if mu is None else float(mu))
+        if not isinstance(mu, SeparableField):
+            raise TypeError("the diffusivity must be a SeparableField or a constant")
+        s, data = mu.tabulate(self.fes.mesh, self.q)
+        L.check(L.lib().gd_plan_set_diffusion(self.handle, C.byref(s), float(sigma), float(kappa)),
+                "gd_plan_set_diffusion")
+
     def set_bc(self, bc):
         if bc is None:
             L.check(L.lib().gd_plan_set_bc(self.handle, L.BC_NONE, None), "gd_plan_set_bc")
@@ -161,8 +173,8 @@ class Plan:
         """Pin the kernel variant: -1 default, 0 generic, 1/2/4 lanes per element
         (register kernel), 17/18/20 (Kronecker kernel, 16 + lanes), 48..55 (LDS-staged
         Kronecker kernel: +1 outputs staged through LDS, +2 128-thread blocks, +4 one term at
-        a time); p-adaptive
-        plans: the layout index."""
+        a time), 56 (streaming form: one wave per block, persistent, next tile's copies in flight);
+        p-adaptive plans: the layout index."""
         nbytes = 8 if dtype == torch.float64 else 4
         L.check(L.lib().gd_plan_set_variant(self.handle, nbytes, lanes), "gd_plan_set_variant")
 
@@ -378,6 +390,43 @@ class AdvectionNonconformingFaceOperator:
         return y
 
     Apply = __call__
+
+
+class DiffusionOperator:
+    """SIPDG diffusion operator (GenDiL's DiffusionOperator,
+    diffusionoperator.hpp:396-477): op(x, y) overwrites y."""
+
+    def __init__(self, plan: Plan):
+        self.plan = plan
+
+    @property
+    def num_dofs(self):
+        return self.plan.num_dofs
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor):
+        out = y
+        x, y = _span(x, self.plan.fes, "x"), _span(y, self.plan.fes, "y")
+        if x.dtype not in (torch.float64, torch.float32):
+            raise TypeError("x must be float64 or float32")
+        _check_vec(x, self.num_dofs, x.dtype, "x")
+        _check_vec(y, self.num_dofs, x.dtype, "y")
+        if _overlap(x, y):
+            raise ValueError("x and y must not overlap")
+        fn = "gd_diff_apply_f64" if x.dtype == torch.float64 else "gd_diff_apply_f32"
+        L.check(getattr(L.lib(), fn)(self.plan.handle, x.data_ptr(), y.data_ptr(), _stream()), fn)
+        return out
+
+    Apply = __call__
+
+
+def MakeDiffusionOperator(fes, ir, velocity, sigma: float, kappa: float) -> DiffusionOperator:
+    """MakeDiffusionOperator(fes, int_rule, velocity, sigma, kappa)
+    (diffusionoperator.hpp:519-537): `velocity` is the diffusivity mu(X) (a
+    SeparableField or a constant), sigma the symmetry parameter (-1 SIPG,
+    0 IIPG, +1 NIPG), kappa the penalty factor (GenDiL's tests use (p+1)^2)."""
+    plan = Plan(fes, ir)
+    plan.set_diffusion(velocity, sigma, kappa)
+    return DiffusionOperator(plan)
 
 
 def MakeAdvectionFaceOperator(*args):

@@ -29,6 +29,27 @@ def advection_c2() -> VectorField:
     )
 
 
+def diffusivity_trig_3d() -> SeparableField:
+    """2 + 0.25 sin 2 pi x + 0.17 cos 2 pi (2y + 0.13) + 0.11 sin 2 pi (3z + 0.37)
+    + 0.07 sin 2 pi (x + 2y + 3z): the diffusivity of GenDiL's SIPDG test
+    (test-diffusion-3d-generic.cpp:121-133) without its last term, whose
+    expansion would need more separable terms than a field holds.  The last
+    sine is expanded as sin(A + B + C) with A = 2 pi x, B = 4 pi y, C = 6 pi z."""
+    sA, cA = (lambda s: np.sin(TWO_PI * s)), (lambda s: np.cos(TWO_PI * s))
+    sB, cB = (lambda s: np.sin(2 * TWO_PI * s)), (lambda s: np.cos(2 * TWO_PI * s))
+    sC, cC = (lambda s: np.sin(3 * TWO_PI * s)), (lambda s: np.cos(3 * TWO_PI * s))
+    return SeparableField([
+        (2.0, {}),
+        (0.25, {0: sA}),
+        (0.17, {1: lambda s: np.cos(TWO_PI * (2.0 * s + 0.13))}),
+        (0.11, {2: lambda s: np.sin(TWO_PI * (3.0 * s + 0.37))}),
+        (0.07, {0: sA, 1: cB, 2: cC}),
+        (0.07, {0: cA, 1: sB, 2: cC}),
+        (0.07, {0: cA, 1: cB, 2: sC}),
+        (-0.07, {0: sA, 1: sB, 2: sC}),
+    ])
+
+
 def advection_c2_max_norm() -> float:
     # |beta|^2 = (1+0.5 s1)^2 + (0.5+0.5 c)^2 + (0.25+0.25 s2)^2, maximised at
     # s1 = 1, c = 1, s2 = 1: 2.25 + 1 + 0.25 = 3.5

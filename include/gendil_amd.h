@@ -120,6 +120,13 @@ int gd_plan_set_beta_pointwise_f64(gd_plan* plan, const double* beta_cell,
                                    const double* const* beta_face);
 /* inflow boundary: mode gd_bc_mode, g separable (may be NULL for g = 0) */
 int gd_plan_set_bc(gd_plan* plan, int32_t mode, const gd_separable* g);
+/* SIPDG diffusion coefficients (DiffusionOperator constructor and
+ * MakeDiffusionOperator, diffusionoperator.hpp:441-455, 519-537):
+ * diffusivity mu(X) separable (NULL = 1; GenDiL's `velocity` argument),
+ * sigma the symmetry parameter (-1 symmetric, 0 incomplete, +1
+ * non-symmetric interior penalty), kappa the penalty factor.  Uniform-order
+ * plans only (GD_ERR_UNSUPPORTED on p-adaptive ones). */
+int gd_plan_set_diffusion(gd_plan* plan, const gd_separable* mu, double sigma, double kappa);
 
 /* ---- operator apply ---------------------------------------------------- */
 /* y = mass_coef * M(sigma) x + adv_coef * A(beta, g) x, y overwritten (L2
@@ -135,6 +142,17 @@ int gd_mass_apply_f32(const gd_plan* plan, const float* x, float* y, gd_stream s
 /* AdvectionOperator::operator() (advectionoperator.hpp:1070) */
 int gd_adv_apply_f64(const gd_plan* plan, const double* u, double* y, gd_stream stream);
 int gd_adv_apply_f32(const gd_plan* plan, const float* u, float* y, gd_stream stream);
+/* y = A_diff x, overwritten (DiffusionOperator::operator(),
+ * diffusionoperator.hpp:469-476; volume DiffusionElementOperator :44-101,
+ * faces DiffusionFaceOperator :130-331): every element's volume term plus
+ * the symmetric-interior-penalty terms of its 2*dim faces; faces on a
+ * non-periodic boundary carry the weak homogeneous Dirichlet terms.  Needs
+ * gd_plan_set_diffusion; x and y must not overlap.  Compiled (nodes,
+ * points): 1D (2,3) (2,4) (3,4) (4,4) (4,5) (5,6); 2D (2,3) (2,4) (3,4)
+ * (3,5) (4,4) (4,5) (4,6); 3D (2,3) (2,4) (3,3) (3,4) (3,5) (4,4) (4,5)
+ * (4,6); 6D (2,3); others return GD_ERR_UNSUPPORTED. */
+int gd_diff_apply_f64(const gd_plan* plan, const double* x, double* y, gd_stream stream);
+int gd_diff_apply_f32(const gd_plan* plan, const float* x, float* y, gd_stream stream);
 /* Adjoint of the linear part, z = (mass_coef M + adv_coef A)^T r.  No GenDiL
  * counterpart (GenDiL has no operator adjoint). */
 int gd_apply_T_f64(const gd_plan* plan, double mass_coef, double adv_coef,
@@ -195,7 +213,10 @@ int gd_adv_hmesh_apply_f64(const gd_plan* coarse, const gd_plan* fine, int32_t f
  * the register-resident 3D kernel with 1, 2 or 4 lanes per element, and the
  * Kronecker-factored 3D kernel (16 + lanes per element: 17, 18, 20) and its
  * LDS-staged form (48..55: 48 + 1 outputs staged through LDS, + 2 blocks of
- * 128 threads, + 4 one term at a time) where they apply.  p-adaptive plans:
+ * 128 threads, + 4 one term at a time) and its streaming form (56: one wave
+ * per block, persistent over tiles of 16 elements of one x-row, the next
+ * tile's copies into LDS in flight while a tile computes; rows must be a
+ * multiple of 16 elements) where they apply.  p-adaptive plans:
  * the thread-group layouts of the quadrature kernel (0 = one wave per n=3
  * element, 1 = one block per n=3 element; n=4 elements take a block, n=2
  * elements a wave) and, in 6D for a field with paired affine components of
@@ -206,7 +227,7 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
                      int32_t reps, int32_t* chosen, double* best_ms);
 /* Pin a variant: -1 (default); uniform plans 0 (generic), 1, 2 or 4 lanes
  * (register kernel), 17, 18 or 20 (Kronecker kernel), 48..55 (LDS-staged
- * Kronecker kernel); p-adaptive plans the variant index as reported by
+ * Kronecker kernel), 56 (streaming Kronecker kernel); p-adaptive plans the variant index as reported by
  * gd_plan_autotune.  A pinned variant that does not serve a later apply
  * falls back to the default for that apply. */
 int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes);

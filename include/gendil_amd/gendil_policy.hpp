@@ -162,8 +162,8 @@ int64_t shift_of(const FES& fes)
 class Mi355xOperator {
  public:
   template <typename FES, typename IR>
-  Mi355xOperator(const FES& fes, const IR&, double mass_coef, double adv_coef)
-      : mass_coef_(mass_coef), adv_coef_(adv_coef), shift_(mi355x_detail::shift_of(fes))
+  Mi355xOperator(const FES& fes, const IR&, double mass_coef, double adv_coef, bool diffusion = false)
+      : mass_coef_(mass_coef), adv_coef_(adv_coef), diffusion_(diffusion), shift_(mi355x_detail::shift_of(fes))
   {
     static_assert(mi355x_detail::space_traits<FES>::dim == mi355x_detail::rule_traits<IR>::dim,
                   "space and rule dimensions differ");
@@ -175,6 +175,8 @@ class Mi355xOperator {
   gendil_amd::Plan& plan() const { return *plan_; }
   double mass_coef() const { return mass_coef_; }
   double adv_coef() const { return adv_coef_; }
+  /// The SIPDG diffusion operator rather than mass_coef M + adv_coef A.
+  bool diffusion() const { return diffusion_; }
   int64_t num_dofs() const { return n_; }
   int64_t shift() const { return shift_; }
   gd_stream stream() const { return stream_; }
@@ -186,15 +188,12 @@ class Mi355xOperator {
     check_size(in);
     check_size(out);
 #ifdef GENDIL_USE_DEVICE
-    gendil_amd::check(gd_apply_f64(plan_->get(), mass_coef_, adv_coef_, in.ReadDeviceData() + shift_,
-                                   out.WriteDeviceData() + shift_, stream_),
-                      "gd_apply_f64");
+    apply(in.ReadDeviceData() + shift_, out.WriteDeviceData() + shift_);
 #else
     // GenDiL built without a device: its vectors live in host memory
     stage();
     to_device(in, dx_);
-    gendil_amd::check(gd_apply_f64(plan_->get(), mass_coef_, adv_coef_, dx_.data(), dy_.data(), stream_),
-                      "gd_apply_f64");
+    apply(dx_.data(), dy_.data());
     to_host(dy_, out);
 #endif
   }
@@ -218,6 +217,11 @@ class Mi355xOperator {
   }
 
  private:
+  void apply(const double* x, double* y) const
+  {
+    if (diffusion_) gendil_amd::check(gd_diff_apply_f64(plan_->get(), x, y, stream_), "gd_diff_apply_f64");
+    else gendil_amd::check(gd_apply_f64(plan_->get(), mass_coef_, adv_coef_, x, y, stream_), "gd_apply_f64");
+  }
   void check_size(const Vector& v) const
   {
     if ((int64_t)v.Size() < shift_ + n_)
@@ -247,6 +251,7 @@ class Mi355xOperator {
 
   std::unique_ptr<gendil_amd::Plan> plan_;
   double mass_coef_, adv_coef_;
+  bool diffusion_ = false;
   int64_t shift_ = 0, n_ = 0;
   gd_stream stream_ = nullptr;
   mutable gendil_amd::DeviceArray<double> dx_, dy_;
@@ -381,6 +386,29 @@ Mi355xOperator MakeMassAdvectionOperator(const FES& fes, const IR& ir, gendil_am
   return op;
 }
 
+/// MakeDiffusionOperator<Mi355xKernelConfiguration>(fes, ir, velocity, sigma,
+/// kappa) (diffusionoperator.hpp:519-537): SIPDG with diffusivity `velocity`,
+/// symmetry parameter sigma and penalty kappa.
+template <typename KernelPolicy, typename FES, typename IR>
+  requires std::is_same_v<KernelPolicy, Mi355xKernelConfiguration>
+Mi355xOperator MakeDiffusionOperator(const FES& fes, const IR& ir, gendil_amd::SeparableField velocity,
+                                     const Real sigma, const Real kappa)
+{
+  Mi355xOperator op(fes, ir, 0.0, 0.0, true);
+  op.plan().set_diffusion(velocity, sigma, kappa);
+  return op;
+}
+
+template <typename KernelPolicy, typename FES, typename IR, typename Velocity>
+  requires(std::is_same_v<KernelPolicy, Mi355xKernelConfiguration> && !mi355x_all_descriptors<Velocity>)
+auto MakeDiffusionOperator(const FES&, const IR&, Velocity&, const Real, const Real)
+{
+  static_assert(mi355x_all_descriptors<Velocity>,
+                "Mi355xKernelConfiguration takes the diffusivity as a gendil_amd::SeparableField descriptor: device "
+                "lambdas cannot cross the C-ABI");
+  return 0;
+}
+
 // ---- Krylov solvers: GenDiL's signatures, device-resident loops ------------
 // GenDiL's solvers are templates on the operator; these overloads, more
 // specialised in the operator, run the library's device-resident loops (no
@@ -392,12 +420,23 @@ Mi355xOperator MakeMassAdvectionOperator(const FES& fes, const IR& ir, gendil_am
 // GenDiL's solvers), which lets callers see which loop ran.
 struct Mi355xSolverResult : IterativeSolverResult {};
 
+namespace mi355x_detail {
+// the device loops apply mass_coef M + adv_coef A of the plan
+inline void require_mass_advection(const Mi355xOperator& A, const char* solver)
+{
+  if (A.diffusion())
+    throw gendil_amd::Error(GD_ERR_UNSUPPORTED, std::string(solver) +
+                                                    ": the device loops run the mass/advection apply, not diffusion");
+}
+}  // namespace mi355x_detail
+
 /// GMRES_no_alloc (gmres.hpp:66-288): restarted GMRES(restart), all FP64.
 template <typename RHSType, typename DotType, typename VectorArray, typename VectorT, typename Array>
 Mi355xSolverResult GMRES_no_alloc(const Mi355xOperator& A, const RHSType& rhs, const DotType&, const Integer max_iters,
                                      const Integer restart, const Real tol, VectorT& x, VectorArray&, VectorT&,
                                      Array&, Array&, Array&, Array&, Array&)
 {
+  mi355x_detail::require_mass_advection(A, "GMRES_no_alloc");
   gd_solve_params prm{(int32_t)restart, (int32_t)max_iters, tol, 100000, 0.0};
   gd_solve_result res{};
   A.with_device(rhs, x, [&](const double* b, double* xd) {
@@ -414,6 +453,7 @@ Mi355xSolverResult BiCGSTAB_no_alloc(const Mi355xOperator& A, const RHSType& rhs
                                         Real tol, VectorT& x, VectorT&, VectorT&, VectorT&, VectorT&, VectorT&,
                                         VectorT&)
 {
+  mi355x_detail::require_mass_advection(A, "BiCGSTAB_no_alloc");
   gd_solve_params prm{30, (int32_t)max_iter, tol, 1, 0.0};
   gd_solve_result res{};
   A.with_device(rhs, x, [&](const double* b, double* xd) {

@@ -8,7 +8,8 @@
 //
 // Factories mirror MakeMassFiniteElementOperator (massoperator.hpp:270),
 // MakeAdvectionOperator with and without inflow data (advectionoperator.hpp:
-// 1416-1528) and MakeMassAdvectionOperator (massadvectionoperator.hpp:447).
+// 1416-1528), MakeMassAdvectionOperator (massadvectionoperator.hpp:447) and
+// MakeDiffusionOperator (diffusionoperator.hpp:519-537).
 // GenDiL's coefficient lambdas cannot cross the C-ABI, so sigma, beta and g
 // are descriptors: sums of products of 1D functions (tabulated here at the
 // plan's sample positions), affine fields, or constants.  Vectors live on
@@ -271,6 +272,12 @@ class Plan {
     check(gd_plan_set_bc(p_, GD_BC_INFLOW, &t.desc), "gd_plan_set_bc");
   }
   void set_no_boundary_terms() { check(gd_plan_set_bc(p_, GD_BC_NONE, nullptr), "gd_plan_set_bc"); }
+  void set_diffusion(const SeparableField& mu, double sigma, double kappa)
+  {
+    Tabulated t = tabulate(mu);
+    t.desc.data = t.data.empty() ? nullptr : t.data.data();
+    check(gd_plan_set_diffusion(p_, &t.desc, sigma, kappa), "gd_plan_set_diffusion");
+  }
 
  private:
   struct Tabulated {
@@ -425,6 +432,47 @@ Operator MakeMassAdvectionOperator(const FiniteElementSpace<Dim>& fes, const Int
   plan->set_sigma(sigma);
   plan->set_beta(beta);
   return Operator(std::move(plan), mass_coef, adv_coef);
+}
+
+// ---- SIPDG diffusion ----------------------------------------------------------
+
+// y = A_diff x on device vectors (DiffusionOperator::operator(),
+// diffusionoperator.hpp:469-476), y overwritten.
+class DiffusionOperator {
+ public:
+  explicit DiffusionOperator(std::shared_ptr<Plan> plan) : plan_(std::move(plan)) {}
+  void operator()(const double* x, double* y) const
+  {
+    check(gd_diff_apply_f64(plan_->get(), x, y, stream_), "gd_diff_apply_f64");
+  }
+  void operator()(const float* x, float* y) const
+  {
+    check(gd_diff_apply_f32(plan_->get(), x, y, stream_), "gd_diff_apply_f32");
+  }
+  template <typename T>
+  void operator()(const Vector<T>& x, Vector<T>& y) const
+  {
+    (*this)(x.ReadDeviceData(), y.WriteDeviceData());
+  }
+  int64_t num_dofs() const { return plan_->num_dofs(); }
+  void SetStream(gd_stream s) { stream_ = s; }
+
+ private:
+  std::shared_ptr<Plan> plan_;
+  gd_stream stream_ = nullptr;
+};
+
+// MakeDiffusionOperator(fes, int_rule, velocity, sigma, kappa)
+// (diffusionoperator.hpp:519-537): `velocity` is the diffusivity, sigma the
+// symmetry parameter (-1 symmetric, 0 incomplete, +1 non-symmetric interior
+// penalty), kappa the penalty factor.
+template <int Dim>
+DiffusionOperator MakeDiffusionOperator(const FiniteElementSpace<Dim>& fes, const IntegrationRule& ir,
+                                        const SeparableField& velocity, double sigma, double kappa)
+{
+  auto plan = std::make_shared<Plan>(fes, ir);
+  plan->set_diffusion(velocity, sigma, kappa);
+  return DiffusionOperator(std::move(plan));
 }
 
 // ---- face-centric operators (accumulate into y) --------------------------------

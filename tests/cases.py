@@ -410,21 +410,7 @@ def diffusivity_field(kind: str, D: int):
         )
     if kind == "trig":
         if D == 3:
-            # 2 + 0.25 sin(2 pi x) + 0.17 cos(2 pi (2y + 0.13)) + 0.11 sin(2 pi (3z + 0.37))
-            #   + 0.07 sin(A + B + C), A = 2 pi x, B = 4 pi y, C = 6 pi z, expanded
-            sA, cA = (lambda s: np.sin(TWO_PI * s)), (lambda s: np.cos(TWO_PI * s))
-            sB, cB = (lambda s: np.sin(2 * TWO_PI * s)), (lambda s: np.cos(2 * TWO_PI * s))
-            sC, cC = (lambda s: np.sin(3 * TWO_PI * s)), (lambda s: np.cos(3 * TWO_PI * s))
-            return SeparableField([
-                (2.0, {}),
-                (0.25, {0: sA}),
-                (0.17, {1: lambda s: np.cos(TWO_PI * (2.0 * s + 0.13))}),
-                (0.11, {2: lambda s: np.sin(TWO_PI * (3.0 * s + 0.37))}),
-                (0.07, {0: sA, 1: cB, 2: cC}),
-                (0.07, {0: cA, 1: sB, 2: cC}),
-                (0.07, {0: cA, 1: cB, 2: sC}),
-                (-0.07, {0: sA, 1: sB, 2: sC}),
-            ])
+            return W.diffusivity_trig_3d()
         last = lambda s: np.cos(TWO_PI * (2.0 * s + 0.13))  # noqa: E731
         if D == 1:
             return SeparableField([(2.0, {}), (0.25, {0: lambda s: np.sin(TWO_PI * s)}), (0.17, {0: last})])

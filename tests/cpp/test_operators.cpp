@@ -249,6 +249,37 @@ int main()
       const double e = rel_l2(y1.ToHost(), y2.ToHost());
       expect(e == 0.0, "advection on the product mesh = on the Cartesian mesh", e);
     }
+    // ---- SIPDG diffusion, 5^3 Dirichlet, p = 3, q = 5, mu = 2 + sin 2 pi x ----
+    {
+      auto mesh = ga::MakeUnitCartesianMesh<3>(5, false);
+      auto fes = ga::MakeFiniteElementSpace(mesh, ga::MakeLobattoFiniteElement(3));
+      auto ir = ga::MakeIntegrationRule(5);
+      ga::SeparableField mu = ga::SeparableField::Constant(2.0);
+      mu.terms.push_back(ga::SeparableField::OneAxis(0, [](double x) { return std::sin(kTwoPi * x); }).terms[0]);
+      auto Ad = ga::MakeDiffusionOperator(fes, ir, mu, -1.0, 16.0);
+      const int64_t n = Ad.num_dofs();
+      const std::vector<double> xh = ga::FillRandomHost(n, 11);
+      ga::Vector<double> x(n), y(n);
+      x.FromHost(xh);
+      Ad(x, y);
+      OracleC2 ref(mesh, 3, 5, 0.0, 0.0, GDO_BC_NONE);
+      std::vector<double> pts((size_t)mesh.sizes[0] * 7);
+      gdo_axis_points(&ref.p, 0, pts.data());
+      std::vector<double> tab;
+      for (double v : pts) tab.push_back(std::sin(kTwoPi * v));
+      gdo_separable m{};
+      m.rank = 2;
+      m.scale[0] = 2.0;
+      m.scale[1] = 1.0;
+      for (int r = 0; r < GDO_MAX_TERMS; ++r)
+        for (int d = 0; d < GDO_MAX_DIM; ++d) m.offset[r][d] = -1;
+      m.offset[1][0] = 0;
+      m.data = tab.data();
+      std::vector<double> yr((size_t)n);
+      gdo_diffusion_apply(&ref.p, &m, -1.0, 16.0, xh.data(), yr.data());
+      const double e = rel_l2(y.ToHost(), yr);
+      expect(e <= 1e-12, "SIPDG diffusion 5^3 p=3 q=5 FP64 vs oracle (rel l2)", e);
+    }
     // ---- errors cross the ABI as exceptions ----
     {
       bool threw = false;

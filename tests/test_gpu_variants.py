@@ -1,6 +1,6 @@
 """Every kernel variant (generic LDS-staged kernel, register-resident 3D
 kernel with 1, 2 or 4 lanes per element, Kronecker-factored 3D kernel with
-16 + lanes, and its LDS-staged forms 48..55) against GenDiL's golden vectors
+16 + lanes, its LDS-staged forms 48..55 and streaming form 56) against GenDiL's golden vectors
 and the oracle, and the autotuner's choice.  A pinned variant that does not
 serve a plan falls back to one that does, so every case must still be
 correct."""
@@ -20,7 +20,7 @@ ELIGIBLE = ["adv3d_c2_p3q4_per", "adv3d_c2_p3q4_bc0", "adv3d_c2_p3q4_nobc", "adv
             "adv3d_bubble_p2q4", "adv3d_bubble_p3q5"]
 STAGED = list(range(48, 56))
 KRON = {1: [17, 18] + STAGED, 2: [17], 3: [18, 20] + STAGED}
-ALL_VARIANTS = (0, 1, 2, 4, 17, 18, 20, *STAGED)
+ALL_VARIANTS = (0, 1, 2, 4, 17, 18, 20, *STAGED, 56)
 
 
 def variants(case):
@@ -44,7 +44,7 @@ def test_all_variants_match_golden(name, dtype):
         assert err <= (1e-12 if dtype == torch.float64 else 1e-5), (lanes, err)
 
 
-@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 18, 20, *STAGED])
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 18, 20, *STAGED, 56])
 def test_variants_combined_operator_at_16_cubed(lanes):
     mesh = G.CartesianMesh.unit(3, 16, False)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
@@ -83,6 +83,11 @@ def test_kronecker_kernel_is_the_default_for_the_benchmark_field():
     # FP32 in 128-thread blocks (+ 2)
     assert op.plan.query_variant(torch.float64, 0.0, 1.0) == 49
     assert op.plan.query_variant(torch.float32, 0.0, 1.0) == 51
+    # rows of a multiple of 16 elements: the streaming kernel
+    fes16 = G.MakeFiniteElementSpace(G.CartesianMesh.unit(3, 16, True), G.MakeLobattoFiniteElement(3))
+    op16 = G.MakeAdvectionOperator(fes16, G.MakeIntegrationRule(4), W.advection_c2())
+    assert op16.plan.query_variant(torch.float64, 0.0, 1.0) == 56
+    assert op16.plan.query_variant(torch.float32, 0.0, 1.0) == 56
     # a field whose sign changes along a face is served by the register kernel
     bad = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(4), beta_field("trig", 3))
     assert bad.plan.query_variant(torch.float64, 0.0, 1.0) in (1, 2, 4)
@@ -177,3 +182,52 @@ def test_staged_kernel_unaligned_vectors_fall_back():
     op(xb[1:], yb[1:])
     assert rel_l2(yb[1:].cpu().numpy(), ref) <= 1e-12
     assert float(yb[0]) == 0.0
+
+
+STREAM_MESHES = [
+    # (sizes, periodic): a grid that is not a multiple of the 8 XCDs (13
+    # tiles), single-element y / z axes, and a mesh of 4096 tiles, several
+    # per wave, so every wave runs the double-buffered loop
+    ((16, 13, 1), (False, True, True)),
+    ((32, 1, 5), (True, False, False)),
+    ((48, 3, 2), (True, True, False)),
+    ((64, 64, 16), (False, False, True)),
+]
+
+
+@pytest.mark.parametrize("mesh_case", STREAM_MESHES, ids=["x16y13", "y1", "x48", "tiles4096"])
+@pytest.mark.parametrize("kind", ["c2", "negc2", "const"])
+def test_streaming_kernel_vs_oracle(mesh_case, kind):
+    """Variant 56 (one wave per block, persistent over tiles of 16 elements of
+    one x-row, copies of the next tile in flight while a tile computes)."""
+    sizes, periodic = mesh_case
+    mesh = G.CartesianMesh(sizes, [1.0 / s for s in sizes], [0.0] * 3, periodic)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    ir = G.MakeIntegrationRule(4)
+    adv = negc2() if kind == "negc2" else beta_field(kind, 3)
+    for mc, ac, bc in ((0.0, 1.0, None), (0.6, 0.8, 0.0), (1.0, 0.0, None)):
+        op = G.MakeMassAdvectionOperator(fes, ir, adv, bc=bc, mass_coef=mc, adv_coef=ac)
+        x = G.fill_random(op.num_dofs, 23)
+        ref = oracle.Problem(fes, ir, mass_coef=mc, adv_coef=ac, adv=adv, bc=bc).apply(x)
+        for dtype in (torch.float64, torch.float32):
+            print(f"stream {sizes} {kind} m={mc} a={ac} {dtype}", flush=True)
+            op.plan.set_variant(dtype, 56)
+            assert op.plan.query_variant(dtype, mc, ac) == 56
+            y = torch.full((op.num_dofs,), float("nan"), dtype=dtype, device="cuda")
+            op(torch.as_tensor(x, dtype=dtype, device="cuda"), y)
+            err = rel_l2(y.double().cpu().numpy(), ref)
+            assert err <= (1e-12 if dtype == torch.float64 else 1e-5), (sizes, kind, mc, dtype, err)
+
+
+def test_streaming_kernel_needs_whole_tiles():
+    """Rows that are not a multiple of 16 elements are served by another variant."""
+    mesh = G.CartesianMesh([20, 4, 4], [0.05, 0.25, 0.25], [0.0] * 3, True)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    op = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(4), W.advection_c2())
+    op.plan.set_variant(torch.float64, 56)
+    assert op.plan.query_variant(torch.float64, 0.0, 1.0) != 56
+    x = G.fill_random(op.num_dofs, 2)
+    y = torch.empty(op.num_dofs, dtype=torch.float64, device="cuda")
+    op(torch.as_tensor(x, device="cuda"), y)
+    ref = oracle.Problem(fes, G.MakeIntegrationRule(4), adv_coef=1.0, adv=W.advection_c2()).apply(x)
+    assert rel_l2(y.cpu().numpy(), ref) <= 1e-12
