@@ -124,6 +124,7 @@ SIGNATURES = {
     "gd_stream_query": (C.c_int, [_p, C.POINTER(C.c_int32)]),
     "gd_device_sync": (C.c_int, []),
     "gd_multidot_f64": (C.c_int, [_i64, _i32, _p, _i64, _p, _p, _p]),
+    "gd_multidot_f64_noalloc": (C.c_int, [_i64, _i32, _p, _i64, _p, _p, _p]),
     "gd_multidot_f32": (C.c_int, [_i64, _i32, _p, _i64, _p, _p, _p]),
     "gd_multiaxpy_f64": (C.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),
     "gd_multiaxpy_f32": (C.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "gd_internal.hpp"
 
@@ -188,6 +189,34 @@ int multiaxpy(int64_t n, int k, const double* h, const TV* V, int64_t ldv, TW* w
   return GD_OK;
 }
 
+// ---- allocation-free multi-dot --------------------------------------------
+// Block partials of gd_multidot_f64_noalloc: one fixed array per device,
+// holding at most kMaxBlocks x kMaxK doubles (256 KB), kept until the process
+// ends.  Its name is a host-side stand-in: host code uses only the device
+// address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch_partials[kMaxBlocks * kMaxK];
+
+constexpr int kMaxDevices = 64;
+
+// The device address of scratch_partials on the current device, looked up
+// once per device (the array has its own address on every GPU).
+int noalloc_partials(double** out)
+{
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  int dev = 0;
+  GD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(GD_ERR_UNSUPPORTED, "multidot_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    GD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch_partials)));
+    addr[dev] = static_cast<double*>(p);
+  }
+  *out = addr[dev];
+  return GD_OK;
+}
+
 }  // namespace blas
 
 }  // namespace gd
@@ -200,6 +229,28 @@ int gd_multidot_f64(int64_t n, int32_t k, const double* V, int64_t ldv, const do
                     gd_stream stream)
 {
   return gd::blas::multidot<double, double>(n, k, V, ldv, w, result, (hipStream_t)stream);
+}
+
+int gd_multidot_f64_noalloc(int64_t n, int32_t k, const double* V, int64_t ldv, const double* w, double* result,
+                            gd_stream stream)
+{
+  using namespace gd::blas;
+  if (k > kMaxK)
+    return fail(GD_ERR_INVALID, "gd_multidot_f64_noalloc: at most 32 vectors per call (its fixed scratch holds the "
+                                "partials of 32)");
+  if (n < 0 || k < 0 || (k > 0 && (!V || !w || !result)))
+    return fail(GD_ERR_INVALID, "gd_multidot_f64_noalloc: bad arguments");
+  if (k == 0) return GD_OK;
+  double* part = nullptr;
+  if (int rc = noalloc_partials(&part)) return rc;
+  // the same kernels as gd_multidot_f64; every partial is written before the
+  // finishing kernel reads it, so nothing needs clearing between calls
+  const int nb = grid_for(n);
+  hipLaunchKernelGGL((k_multidot<double, double>), dim3(nb), dim3(kThreads), 0, (hipStream_t)stream, n, k, V, ldv, w,
+                     part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kThreads), 0, (hipStream_t)stream, nb, k, part, result);
+  GD_HIP(hipGetLastError());
+  return GD_OK;
 }
 
 int gd_multidot_f32(int64_t n, int32_t k, const float* V, int64_t ldv, const float* w, double* result,

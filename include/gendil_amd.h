@@ -258,6 +258,17 @@ int gd_multidot_f64(int64_t n, int32_t k, const double* V, int64_t ldv, const do
                     double* result, gd_stream stream);
 int gd_multidot_f32(int64_t n, int32_t k, const float* V, int64_t ldv, const float* w,
                     double* result, gd_stream stream);
+/* gd_multidot_f64 without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same kernels and the
+ * same result bit for bit; the block partials live in a fixed array of the
+ * library (256 KB per GPU, held until the process ends) instead of a buffer
+ * grown with hipMalloc.  At most 32 vectors per call (k > 32 returns
+ * GD_ERR_INVALID).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (on different streams, or from different threads) race
+ * and must be ordered by the caller. */
+int gd_multidot_f64_noalloc(int64_t n, int32_t k, const double* V, int64_t ldv, const double* w,
+                            double* result, gd_stream stream);
 /* w -= sum_j h[j] * V_j  (h device, FP64) */
 int gd_multiaxpy_f64(int64_t n, int32_t k, const double* h, const double* V, int64_t ldv,
                      double* w, gd_stream stream);
