@@ -77,7 +77,7 @@ def __getattr__(name):
         from . import operators
 
         return getattr(operators, name)
-    if name in ("gmres", "gmres_mixed", "cg", "SolveResult"):
+    if name in ("gmres", "gmres_mixed", "cg", "cg_mixed", "bicgstab", "SolveResult"):
         from . import solvers
 
         return getattr(solvers, name)

@@ -148,10 +148,13 @@ SIGNATURES = {
     "gd_gmres_f64": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),
     "gd_gmres_mixed": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),
     "gd_cg_f64": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),
+    "gd_cg_mixed": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),
     "gd_bicgstab_f64": (C.c_int, [_p, _d, _d, _p, _p, C.POINTER(GdSolveParams), C.POINTER(GdSolveResult), _p]),
     "gd_ksolve_workspace_bytes": (C.c_size_t, []),
     "gd_cg_begin_f64": (C.c_int, [_i64, _p, _d, _d, _p, _p]),
     "gd_cg_step_f64": (C.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p]),
+    "gd_cg_begin_f32": (C.c_int, [_i64, _p, _d, _d, _p, _p]),
+    "gd_cg_step_f32": (C.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p]),
     "gd_bicgstab_begin_f64": (C.c_int, [_i64, _p, _p, _d, _d, _p, _p]),
     "gd_bicgstab_p_f64": (C.c_int, [_i64, _p, _p, _p, _p, _p]),
     "gd_bicgstab_s_f64": (C.c_int, [_i64, _p, _p, _p, _p, _p, _p]),

@@ -83,6 +83,18 @@ int gd_cg_f64(const gd_plan* plan, double mass_coef, double adv_coef, const doub
   });
 }
 
+int gd_cg_mixed(const gd_plan* plan, double mass_coef, double adv_coef, const double* b, double* x,
+                const gd_solve_params* params, gd_solve_result* result, gd_stream stream)
+{
+  if (!plan || !b || !x) return gd::fail(GD_ERR_INVALID, "gd_cg_mixed: null argument");
+  return guarded([&] {
+    gendil_amd::CombinedOperator K{plan, mass_coef, adv_coef, stream};
+    auto rep = gendil_amd::cg_mixed_solve(K, gd_plan_num_dofs(plan), b, x, to_options(params));
+    to_result(rep, result);
+    return rep.success ? GD_OK : GD_ERR_NOT_CONVERGED;
+  });
+}
+
 int gd_bicgstab_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* b, double* x,
                     const gd_solve_params* params, gd_solve_result* result, gd_stream stream)
 {

@@ -69,10 +69,11 @@ __device__ __forceinline__ void owe(Workspace* ws, double* resid_out)
 
 // ---- shared passes ---------------------------------------------------------
 
-// partials of <a, b> (and <c, d> when c is not null); skipped once the
-// iteration has ended, except when starting one (begin)
-__global__ void __launch_bounds__(kThreads) k_dots(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
-                                                   const double* __restrict__ c, const double* __restrict__ d,
+// partials of <a, b> (and <c, d> when c is not null), accumulated in FP64;
+// skipped once the iteration has ended, except when starting one (begin)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_dots(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                   const T* __restrict__ c, const T* __restrict__ d,
                                                    Workspace* __restrict__ ws, int begin)
 {
   if (!begin && ws->state != kRunning) return;
@@ -80,13 +81,13 @@ __global__ void __launch_bounds__(kThreads) k_dots(int64_t n, const double* __re
   if (c) {
     GD_KSOLVE_LOOP(i)
     {
-      acc[0] += a[i] * b[i];
-      acc[1] += c[i] * d[i];
+      acc[0] += (double)a[i] * (double)b[i];
+      acc[1] += (double)c[i] * (double)d[i];
     }
     red::store_block_partials<2>(acc, ws->partial);
   } else {
     double a1[1] = {0.0};
-    GD_KSOLVE_LOOP(i) a1[0] += a[i] * b[i];
+    GD_KSOLVE_LOOP(i) a1[0] += (double)a[i] * (double)b[i];
     red::store_block_partials<1>(a1, ws->partial);
   }
 }
@@ -126,19 +127,21 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_fin1(int nblk, Workspace* __
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_cg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
-                                                    const double* __restrict__ p, const double* __restrict__ ap,
+// vectors in precision T, the scalars and <r, r> in FP64
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_cg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
+                                                    const T* __restrict__ p, const T* __restrict__ ap,
                                                     Workspace* __restrict__ ws)
 {
   if (ws->state != kRunning) return;
-  const double alpha = ws->alpha;
+  const T alpha = (T)ws->alpha;
   double acc[1] = {0.0};
   GD_KSOLVE_LOOP(i)
   {
     x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * ap[i];
+    const T ri = r[i] - alpha * ap[i];
     r[i] = ri;
-    acc[0] += ri * ri;
+    acc[0] += (double)ri * (double)ri;
   }
   red::store_block_partials<1>(acc, ws->partial);
 }
@@ -161,12 +164,13 @@ __global__ void __launch_bounds__(kFinThreads) k_cg_fin2(int nblk, Workspace* __
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_cg_p(int64_t n, const double* __restrict__ r, double* __restrict__ p,
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_cg_p(int64_t n, const T* __restrict__ r, T* __restrict__ p,
                                                    Workspace* __restrict__ ws)
 {
   publish(ws);
   if (ws->state != kRunning) return;
-  const double beta = ws->beta;
+  const T beta = (T)ws->beta;
   GD_KSOLVE_LOOP(i) p[i] = r[i] + beta * p[i];
 }
 
@@ -331,32 +335,65 @@ extern "C" {
 
 size_t gd_ksolve_workspace_bytes(void) { return sizeof(ks::Workspace); }
 
-int gd_cg_begin_f64(int64_t n, const double* r, double bnorm, double rel_tol, void* ws, gd_stream stream)
+}  // extern "C"
+
+namespace gd {
+namespace ksolve {
+
+template <typename T>
+int cg_begin(int64_t n, const T* r, double bnorm, double rel_tol, void* ws, gd_stream stream)
 {
-  if (n < 1 || !r || !ws || !(bnorm > 0.0)) return fail(GD_ERR_INVALID, "gd_cg_begin_f64: bad arguments");
+  if (n < 1 || !r || !ws || !(bnorm > 0.0)) return fail(GD_ERR_INVALID, "gd_cg_begin: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  auto* w = static_cast<ks::Workspace*>(ws);
-  const int nb = ks::blocks(n);
-  hipLaunchKernelGGL(ks::k_dots, dim3(nb), dim3(ks::kThreads), 0, st, n, r, r, nullptr, nullptr, w, 1);
-  hipLaunchKernelGGL(ks::k_cg_fin0, dim3(1), dim3(ks::kFinThreads), 0, st, nb, bnorm, rel_tol, w);
+  auto* w = static_cast<Workspace*>(ws);
+  const int nb = blocks(n);
+  hipLaunchKernelGGL(k_dots<T>, dim3(nb), dim3(kThreads), 0, st, n, r, r, nullptr, nullptr, w, 1);
+  hipLaunchKernelGGL(k_cg_fin0, dim3(1), dim3(kFinThreads), 0, st, nb, bnorm, rel_tol, w);
   GD_HIP(hipGetLastError());
   return GD_OK;
+}
+
+template <typename T>
+int cg_step(int64_t n, T* x, T* r, T* p, const T* ap, void* ws, double* resid_out, gd_stream stream)
+{
+  if (n < 1 || !x || !r || !p || !ap || !ws) return fail(GD_ERR_INVALID, "gd_cg_step: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  auto* w = static_cast<Workspace*>(ws);
+  const int nb = blocks(n);
+  hipLaunchKernelGGL(k_dots<T>, dim3(nb), dim3(kThreads), 0, st, n, p, ap, nullptr, nullptr, w, 0);
+  hipLaunchKernelGGL(k_cg_fin1, dim3(1), dim3(kFinThreads), 0, st, nb, w);
+  hipLaunchKernelGGL(k_cg_xr<T>, dim3(nb), dim3(kThreads), 0, st, n, x, r, p, ap, w);
+  hipLaunchKernelGGL(k_cg_fin2, dim3(1), dim3(kFinThreads), 0, st, nb, w, resid_out);
+  hipLaunchKernelGGL(k_cg_p<T>, dim3(nb), dim3(kThreads), 0, st, n, r, p, w);
+  GD_HIP(hipGetLastError());
+  return GD_OK;
+}
+
+}  // namespace ksolve
+}  // namespace gd
+
+extern "C" {
+
+int gd_cg_begin_f64(int64_t n, const double* r, double bnorm, double rel_tol, void* ws, gd_stream stream)
+{
+  return gd::ksolve::cg_begin<double>(n, r, bnorm, rel_tol, ws, stream);
 }
 
 int gd_cg_step_f64(int64_t n, double* x, double* r, double* p, const double* ap, void* ws, double* resid_out,
                    gd_stream stream)
 {
-  if (n < 1 || !x || !r || !p || !ap || !ws) return fail(GD_ERR_INVALID, "gd_cg_step_f64: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
-  auto* w = static_cast<ks::Workspace*>(ws);
-  const int nb = ks::blocks(n);
-  hipLaunchKernelGGL(ks::k_dots, dim3(nb), dim3(ks::kThreads), 0, st, n, p, ap, nullptr, nullptr, w, 0);
-  hipLaunchKernelGGL(ks::k_cg_fin1, dim3(1), dim3(ks::kFinThreads), 0, st, nb, w);
-  hipLaunchKernelGGL(ks::k_cg_xr, dim3(nb), dim3(ks::kThreads), 0, st, n, x, r, p, ap, w);
-  hipLaunchKernelGGL(ks::k_cg_fin2, dim3(1), dim3(ks::kFinThreads), 0, st, nb, w, resid_out);
-  hipLaunchKernelGGL(ks::k_cg_p, dim3(nb), dim3(ks::kThreads), 0, st, n, r, p, w);
-  GD_HIP(hipGetLastError());
-  return GD_OK;
+  return gd::ksolve::cg_step<double>(n, x, r, p, ap, ws, resid_out, stream);
+}
+
+int gd_cg_begin_f32(int64_t n, const float* r, double bnorm, double rel_tol, void* ws, gd_stream stream)
+{
+  return gd::ksolve::cg_begin<float>(n, r, bnorm, rel_tol, ws, stream);
+}
+
+int gd_cg_step_f32(int64_t n, float* x, float* r, float* p, const float* ap, void* ws, double* resid_out,
+                   gd_stream stream)
+{
+  return gd::ksolve::cg_step<float>(n, x, r, p, ap, ws, resid_out, stream);
 }
 
 int gd_bicgstab_begin_f64(int64_t n, const double* rhat, const double* r, double bnorm, double rel_tol, void* ws,
@@ -366,7 +403,7 @@ int gd_bicgstab_begin_f64(int64_t n, const double* rhat, const double* r, double
   hipStream_t st = (hipStream_t)stream;
   auto* w = static_cast<ks::Workspace*>(ws);
   const int nb = ks::blocks(n);
-  hipLaunchKernelGGL(ks::k_dots, dim3(nb), dim3(ks::kThreads), 0, st, n, rhat, r, r, r, w, 1);
+  hipLaunchKernelGGL(ks::k_dots<double>, dim3(nb), dim3(ks::kThreads), 0, st, n, rhat, r, r, r, w, 1);
   hipLaunchKernelGGL(ks::k_bi_fin0, dim3(1), dim3(ks::kFinThreads), 0, st, nb, bnorm, rel_tol, w);
   GD_HIP(hipGetLastError());
   return GD_OK;
@@ -388,7 +425,7 @@ int gd_bicgstab_s_f64(int64_t n, const double* r, const double* rhat, const doub
   hipStream_t st = (hipStream_t)stream;
   auto* w = static_cast<ks::Workspace*>(ws);
   const int nb = ks::blocks(n);
-  hipLaunchKernelGGL(ks::k_dots, dim3(nb), dim3(ks::kThreads), 0, st, n, rhat, v, nullptr, nullptr, w, 0);
+  hipLaunchKernelGGL(ks::k_dots<double>, dim3(nb), dim3(ks::kThreads), 0, st, n, rhat, v, nullptr, nullptr, w, 0);
   hipLaunchKernelGGL(ks::k_bi_alpha, dim3(1), dim3(ks::kFinThreads), 0, st, nb, w);
   hipLaunchKernelGGL(ks::k_bi_s, dim3(nb), dim3(ks::kThreads), 0, st, n, r, v, s, w);
   hipLaunchKernelGGL(ks::k_bi_snorm, dim3(1), dim3(ks::kFinThreads), 0, st, nb, w);
@@ -404,7 +441,7 @@ int gd_bicgstab_xr_f64(int64_t n, double* x, double* r, const double* rhat, cons
   hipStream_t st = (hipStream_t)stream;
   auto* w = static_cast<ks::Workspace*>(ws);
   const int nb = ks::blocks(n);
-  hipLaunchKernelGGL(ks::k_dots, dim3(nb), dim3(ks::kThreads), 0, st, n, t, s, t, t, w, 0);
+  hipLaunchKernelGGL(ks::k_dots<double>, dim3(nb), dim3(ks::kThreads), 0, st, n, t, s, t, t, w, 0);
   hipLaunchKernelGGL(ks::k_bi_omega, dim3(1), dim3(ks::kFinThreads), 0, st, nb, w);
   hipLaunchKernelGGL(ks::k_bi_xr, dim3(nb), dim3(ks::kThreads), 0, st, n, x, r, rhat, p, s, t, w);
   hipLaunchKernelGGL(ks::k_bi_fin, dim3(1), dim3(ks::kFinThreads), 0, st, nb, w, resid_out);

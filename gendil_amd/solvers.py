@@ -1,5 +1,5 @@
 """Device-resident Krylov solves over the C-ABI (gd_gmres_f64 / gd_gmres_mixed /
-gd_cg_f64 / gd_bicgstab_f64, implemented by the header-only solvers of
+gd_cg_f64 / gd_cg_mixed / gd_bicgstab_f64, implemented by the header-only solvers of
 include/gendil_amd/krylov.hpp).
 
 They solve K x = b for K = mass_coef * M + adv_coef * A of one operator plan,
@@ -54,6 +54,12 @@ def cg(op, b, x, max_iters=10000, rel_tol=1e-10):
     """Unpreconditioned CG (conjugategradient.hpp:217) for SPD operators,
     device-resident (no dot product synchronises the stream)."""
     return _run(L.lib().gd_cg_f64, op, b, x, 30, max_iters, rel_tol, 1, 0.0)
+
+
+def cg_mixed(op, b, x, max_iters=10000, rel_tol=1e-10, inner_rel_tol=1e-6, max_refinements=1000):
+    """CG for SPD operators with FP32 inner solves (FP32 vectors and apply)
+    refined in FP64 on the true residual (gd_cg_mixed)."""
+    return _run(L.lib().gd_cg_mixed, op, b, x, 30, max_iters, rel_tol, max_refinements, inner_rel_tol)
 
 
 def bicgstab(op, b, x, max_iters=10000, rel_tol=1e-10):

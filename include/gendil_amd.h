@@ -333,6 +333,11 @@ size_t gd_ksolve_workspace_bytes(void);
 int gd_cg_begin_f64(int64_t n, const double* r, double bnorm, double rel_tol, void* ws, gd_stream stream);
 int gd_cg_step_f64(int64_t n, double* x, double* r, double* p, const double* ap, void* ws, double* resid_out,
                    gd_stream stream);
+/* The same CG passes on FP32 vectors (inner products, alpha and beta in
+ * FP64): the inner solve of gd_cg_mixed.  No GenDiL counterpart. */
+int gd_cg_begin_f32(int64_t n, const float* r, double bnorm, double rel_tol, void* ws, gd_stream stream);
+int gd_cg_step_f32(int64_t n, float* x, float* r, float* p, const float* ap, void* ws, double* resid_out,
+                   gd_stream stream);
 int gd_bicgstab_begin_f64(int64_t n, const double* rhat, const double* r, double bnorm, double rel_tol, void* ws,
                           gd_stream stream);
 int gd_bicgstab_p_f64(int64_t n, const double* r, double* p, const double* v, void* ws, gd_stream stream);
@@ -388,6 +393,16 @@ int gd_gmres_mixed(const gd_plan* plan, double mass_coef, double adv_coef, const
 int gd_cg_f64(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
               double* x, const gd_solve_params* params, gd_solve_result* result,
               gd_stream stream);
+/* CG with FP32 inner solves refined in FP64: every refinement step forms the
+ * true residual r = b - K x with the FP64 apply, runs CG on K d = r / ||r||
+ * with FP32 vectors and the FP32 apply until its residual falls below
+ * max(inner_rel_tol, rel_tol ||b|| / ||r||) (or max_iters in all), and adds
+ * ||r|| d to x in FP64; stops once the true FP64 residual is at most rel_tol
+ * (or after max_refinements steps).  result->iterations counts the inner
+ * iterations, result->refinements the steps.  No GenDiL counterpart. */
+int gd_cg_mixed(const gd_plan* plan, double mass_coef, double adv_coef, const double* b,
+                double* x, const gd_solve_params* params, gd_solve_result* result,
+                gd_stream stream);
 /* Unpreconditioned BiCGSTAB (BiCGSTAB_no_alloc, bicgstab.hpp:44-157),
  * device-resident (gd_bicgstab_*_f64): the reference's stopping rule
  * (||r|| / ||b|| < rel_tol), early exit, fallback and breakdown guards.

@@ -27,6 +27,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "gendil_amd.h"
@@ -300,6 +301,49 @@ void drive(int max_iters, double rel_tol, bool strict, double* resid, gd_stream 
 
 }  // namespace detail
 
+namespace detail {
+
+inline int cg_begin(int64_t n, const double* r, double bnorm, double tol, void* ws, gd_stream st)
+{
+  return gd_cg_begin_f64(n, r, bnorm, tol, ws, st);
+}
+inline int cg_begin(int64_t n, const float* r, double bnorm, double tol, void* ws, gd_stream st)
+{
+  return gd_cg_begin_f32(n, r, bnorm, tol, ws, st);
+}
+inline int cg_step(int64_t n, double* x, double* r, double* p, const double* ap, void* ws, double* out, gd_stream st)
+{
+  return gd_cg_step_f64(n, x, r, p, ap, ws, out, st);
+}
+inline int cg_step(int64_t n, float* x, float* r, float* p, const float* ap, void* ws, double* out, gd_stream st)
+{
+  return gd_cg_step_f32(n, x, r, p, ap, ws, out, st);
+}
+inline int copy(int64_t n, const double* a, double* b, gd_stream st) { return gd_axpby_f64(n, 1.0, a, 0.0, b, st); }
+inline int copy(int64_t n, const float* a, float* b, gd_stream st) { return gd_axpby_f32(n, 1.0f, a, 0.0f, b, st); }
+
+// The device-resident CG iterations on K x = rhs from the residual r = rhs -
+// K x (r is overwritten, p and ap are scratch), vectors in precision T, until
+// ||r|| <= rel_tol * bnorm or `max_iters`; returns (state, iterations).
+template <typename T>
+std::pair<int32_t, int32_t> cg_iterate(const CombinedOperator& K, int64_t n, T* x, T* r, T* p, T* ap, void* ws,
+                                       double* resid, double bnorm, double rel_tol, int max_iters)
+{
+  const gd_stream st = K.stream;
+  check(copy(n, r, p, st), "copy");
+  check(cg_begin(n, r, bnorm, rel_tol, ws, st), "gd_cg_begin");
+  drive(max_iters, rel_tol, false, resid, st, [&](int, double* out) {
+    K.apply(1.0, p, ap);
+    check(cg_step(n, x, r, p, ap, ws, out, st), "gd_cg_step");
+  });
+  int32_t state = 0, iters = 0;
+  double rel = 0.0;
+  check(gd_ksolve_status(ws, st, &state, &iters, &rel), "gd_ksolve_status");
+  return {state, iters};
+}
+
+}  // namespace detail
+
 // Unpreconditioned CG (conjugategradient.hpp:217) for SPD K, FP64,
 // device-resident (gd_cg_step_f64): no dot product synchronises the stream.
 inline SolveReport cg_solve(const CombinedOperator& K, int64_t n, const double* b, double* x,
@@ -320,18 +364,54 @@ inline SolveReport cg_solve(const CombinedOperator& K, int64_t n, const double* 
   }
   K.apply(1.0, x, r.data());
   check(gd_axpby_f64(n, 1.0, b, -1.0, r.data(), st), "axpby");
-  check(gd_axpby_f64(n, 1.0, r.data(), 0.0, p.data(), st), "copy");
-  check(gd_cg_begin_f64(n, r.data(), bnorm, opt.rel_tol, ws.data(), st), "gd_cg_begin_f64");
-  detail::drive(opt.max_iters, opt.rel_tol, false, resid.data(), st, [&](int, double* out) {
-    K.apply(1.0, p.data(), Ap.data());
-    check(gd_cg_step_f64(n, x, r.data(), p.data(), Ap.data(), ws.data(), out, st), "gd_cg_step_f64");
-  });
-  int32_t state = 0, iters = 0;
-  double rel = 0.0;
-  check(gd_ksolve_status(ws.data(), st, &state, &iters, &rel), "gd_ksolve_status");
+  const auto [state, iters] = detail::cg_iterate<double>(K, n, x, r.data(), p.data(), Ap.data(), ws.data(),
+                                                         resid.data(), bnorm, opt.rel_tol, opt.max_iters);
   rep.iterations = iters;
   rep.success = state == 1;
   rep.final_rel_residual = detail::true_residual(K, n, b, x, r.data(), tmp.data(), bnorm);
+  rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rep;
+}
+
+// CG with FP32 inner solves and FP64 iterative refinement: each step forms
+// the true residual r = b - K x in FP64, solves K d = r / ||r|| by CG with
+// FP32 vectors and the FP32 apply to max(inner_rel_tol, rel_tol ||b|| /
+// ||r||), and updates x += ||r|| d in FP64.
+inline SolveReport cg_mixed_solve(const CombinedOperator& K, int64_t n, const double* b, double* x,
+                                  const SolveOptions& opt)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  const gd_stream st = K.stream;
+  DeviceArray<double> r(n), tmp(4);
+  DeviceArray<float> d(n), r32(n), p(n), Ap(n);
+  DeviceArray<unsigned char> ws(gd_ksolve_workspace_bytes());
+  PinnedArray<double> resid(2);
+  SolveReport rep;
+  const double bnorm = std::sqrt(detail::dev_dot(n, b, b, tmp.data(), st));
+  if (bnorm == 0.0) {
+    check(gd_axpby_f64(n, 0.0, b, 0.0, x, st), "gd_axpby_f64");
+    check(gd_stream_sync(st), "sync");
+    rep.success = true;
+    return rep;
+  }
+  double rel = detail::true_residual(K, n, b, x, r.data(), tmp.data(), bnorm);
+  while (rel > opt.rel_tol && rep.cycles < opt.max_refinements && rep.iterations < opt.max_iters) {
+    const double rnorm = rel * bnorm;
+    // r (FP64, already b - K x) -> r32 = r / ||r||, d = 0
+    check(gd_convert_f64_to_f32(n, 1.0 / rnorm, r.data(), r32.data(), st), "gd_convert_f64_to_f32");
+    check(gd_axpby_f32(n, 0.0f, r32.data(), 0.0f, d.data(), st), "gd_axpby_f32");
+    const double inner = std::max(opt.inner_rel_tol, opt.rel_tol * bnorm / rnorm);
+    const auto [state, iters] =
+        detail::cg_iterate<float>(K, n, d.data(), r32.data(), p.data(), Ap.data(), ws.data(), resid.data(), 1.0,
+                                  inner, opt.max_iters - rep.iterations);
+    rep.iterations += iters;
+    rep.cycles += 1;
+    if (iters == 0 && state != 1) break;  // no progress possible in FP32
+    check(gd_add_f32_to_f64(n, rnorm, d.data(), x, st), "gd_add_f32_to_f64");
+    rel = detail::true_residual(K, n, b, x, r.data(), tmp.data(), bnorm);
+  }
+  rep.success = rel <= opt.rel_tol;
+  rep.final_rel_residual = rel;
   rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return rep;
 }

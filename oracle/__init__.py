@@ -295,9 +295,18 @@ def ref_bench(op, D, P, Q, basis, n, periodic, coef, bc, seed, pairs, env=None, 
 
 def ref_bicgstab(D, P, Q, basis, n, coef, bc, seed, path):
     """GenDiL's BiCGSTAB_no_alloc (bicgstab.hpp:44) on M + 0.05 A, b =
-    FillRandom(seed), x0 = 0, tol 1e-10 (ref_harness.cpp, solve_bicgstab).
+    FillRandom(seed), x0 = 0, tol 1e-10 (ref_common.hpp, solve_bicgstab).
     Returns (b, x, iterations, success, relative residual)."""
-    run_ref(["bicgstab", "madv", D, P, Q, basis, n, 0, coef, bc, seed, path])
+    return ref_solve("bicgstab", D, P, Q, basis, n, coef, bc, seed, path)
+
+
+def ref_solve(mode, D, P, Q, basis, n, coef, bc, seed, path):
+    """GenDiL's own solver `mode` through _ref/gendil_ref (ref_common.hpp):
+    bicgstab (BiCGSTAB_no_alloc) or gmres (GMRES_no_alloc, restart 30) on
+    M + 0.05 A(coef, inflow bc), cg (ConjugateGradient) on M(coef); b =
+    FillRandom(seed), x0 = 0, tol 1e-10, at most 500 iterations.
+    Returns (b, x, iterations, success, relative residual)."""
+    run_ref([mode, "madv", D, P, Q, basis, n, 0, coef, bc, seed, path])
     d = np.fromfile(path, dtype=np.float64)
     nd = int(d[:1].view(np.int64)[0])
     b = d[1 : 1 + nd].copy()

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <string>
 #include <utility>
+#include <vector>
 
 namespace gdref {
 
@@ -210,6 +211,23 @@ int finish( const Args & a, const Op & op, Integer ndofs, Integer nelem )
    return 0;
 }
 
+// Solve fixture: int64 n, b, x, then iterations, success and the returned
+// relative residual.
+inline int write_solve( const Args & a, const Vector & b, const Vector & x, Integer ndofs, double iters, bool ok,
+                        double relres )
+{
+   FILE * f = std::fopen( a.out.c_str(), "wb" );
+   if ( !f ) { std::perror( "fopen" ); return 3; }
+   const long long n = ndofs;
+   std::fwrite( &n, sizeof( n ), 1, f );
+   std::fwrite( b.ReadHostData(), sizeof( Real ), ndofs, f );
+   std::fwrite( x.ReadHostData(), sizeof( Real ), ndofs, f );
+   const double tail[3] = { iters, ok ? 1.0 : 0.0, relres };
+   std::fwrite( tail, sizeof( double ), 3, f );
+   std::fclose( f );
+   return 0;
+}
+
 // GenDiL's own BiCGSTAB_no_alloc (bicgstab.hpp:44) on K = M + dt A (unit mass
 // density, advection field `coef` with boundary field `bc`), b = FillRandom(seed),
 // x0 = 0, tol 1e-10, at most 500 iterations.  Writes n, b, x, iterations,
@@ -236,16 +254,57 @@ int solve_bicgstab( const Args & a, const Space & fes, const Rule & ir, Integer 
    FillRandom( b, a.seed );
    x = 0.0;
    auto [ok, iters, relres] = BiCGSTAB_no_alloc( K, b, dot, 500, 1e-10, x, r, r_hat, p, v, s, t );
-   FILE * f = std::fopen( a.out.c_str(), "wb" );
-   if ( !f ) { std::perror( "fopen" ); return 3; }
-   const long long n = ndofs;
-   std::fwrite( &n, sizeof( n ), 1, f );
-   std::fwrite( b.ReadHostData(), sizeof( Real ), ndofs, f );
-   std::fwrite( x.ReadHostData(), sizeof( Real ), ndofs, f );
-   const double tail[3] = { (double)iters, ok ? 1.0 : 0.0, (double)relres };
-   std::fwrite( tail, sizeof( double ), 3, f );
-   std::fclose( f );
-   return 0;
+   return write_solve( a, b, x, ndofs, (double)iters, ok, (double)relres );
+}
+
+
+// GenDiL's own GMRES_no_alloc (gmres.hpp:66-288) on K = M + dt A as in
+// solve_bicgstab, restart 30, b = FillRandom(seed), x0 = 0, tol 1e-10, at
+// most 500 iterations.  Same output as solve_bicgstab.
+template < int D, typename Space, typename Rule >
+int solve_gmres( const Args & a, const Space & fes, const Rule & ir, Integer ndofs )
+{
+   const std::string coef = a.coef, bc = a.bc;
+   auto sigma = []( const std::array< Real, D > & ) -> Real { return 1.0; };
+   auto adv = [coef]( const std::array< Real, D > & X, Real ( &v )[D] ) { advection_field< D >( coef, X, v ); };
+   auto g = [bc]( const std::array< Real, D > & X ) -> Real { return boundary_value< D >( bc, X ); };
+   auto M = MakeMassFiniteElementOperator< SerialKernelConfiguration >( fes, ir, sigma );
+   auto A = MakeAdvectionOperator< SerialKernelConfiguration >( fes, ir, adv, g );
+   const Real dt = 0.05;
+   Vector tmp( ndofs );
+   auto K = [&]( const Vector & x, Vector & y )
+   {
+      M( x, y );
+      A( x, tmp );
+      Axpy( dt, tmp, y );
+   };
+   auto dot = []( const Vector & u, const Vector & v ) { return Dot( u, v ); };
+   const Integer restart = 30;
+   std::vector< Vector > V;
+   for ( Integer i = 0; i <= restart; ++i ) V.emplace_back( ndofs );
+   std::vector< Real > H( ( restart + 1 ) * restart ), cs( restart ), sn( restart ), e1( restart + 1 ), yv( restart );
+   Vector b( ndofs ), x( ndofs ), w( ndofs );
+   FillRandom( b, a.seed );
+   x = 0.0;
+   auto [ok, iters, relres] = GMRES_no_alloc( K, b, dot, 500, restart, 1e-10, x, V, w, H, cs, sn, e1, yv );
+   return write_solve( a, b, x, ndofs, (double)iters, ok, (double)relres );
+}
+
+// GenDiL's own ConjugateGradient (conjugategradient.hpp:118-208, the variant
+// with an injected dot and scratch vectors) on the mass matrix M(sigma = coef),
+// b = FillRandom(seed), x0 = 0, tol 1e-10, at most 500 iterations.
+template < int D, typename Space, typename Rule >
+int solve_cg( const Args & a, const Space & fes, const Rule & ir, Integer ndofs )
+{
+   const std::string coef = a.coef;
+   auto sigma = [coef]( const std::array< Real, D > & X ) -> Real { return mass_density< D >( coef, X ); };
+   auto M = MakeMassFiniteElementOperator< SerialKernelConfiguration >( fes, ir, sigma );
+   auto dot = []( const Vector & u, const Vector & v ) { return Dot( u, v ); };
+   Vector b( ndofs ), x( ndofs ), tmp( ndofs ), z( ndofs ), r( ndofs ), p( ndofs );
+   FillRandom( b, a.seed );
+   x = 0.0;
+   auto [ok, iters, relres] = ConjugateGradient( M, b, dot, 500, 1e-10, x, tmp, z, r, p );
+   return write_solve( a, b, x, ndofs, (double)iters, ok, (double)relres );
 }
 
 template < int D, int P, int Q, bool Lobatto >
@@ -258,6 +317,8 @@ int run( const Args & a )
    const std::string coef = a.coef;
    const std::string bc = a.bc;
    if ( a.mode == "bicgstab" ) return solve_bicgstab< D >( a, fes, ir, ndofs );
+   if ( a.mode == "gmres" ) return solve_gmres< D >( a, fes, ir, ndofs );
+   if ( a.mode == "cg" ) return solve_cg< D >( a, fes, ir, ndofs );
    if ( a.op == "mass" )
    {
       auto sigma = [coef]( const std::array< Real, D > & X ) -> Real { return mass_density< D >( coef, X ); };

@@ -10,6 +10,11 @@
 // Usage:
 //   gendil_ref fixture <op> <D> <P> <Q> <basis> <n> <periodic> <coef> <bc> <seed> <out.bin>
 //   gendil_ref bench   <op> <D> <P> <Q> <basis> <n> <periodic> <coef> <bc> <seed> <pairs>
+//   gendil_ref bicgstab|gmres <op> ... <out.bin>: GenDiL's BiCGSTAB_no_alloc or
+//              GMRES_no_alloc (restart 30) on M + 0.05 A(coef, inflow bc)
+//   gendil_ref cg <op> ... <out.bin>: GenDiL's ConjugateGradient on M(coef)
+//              (solves: b = FillRandom(seed), x0 = 0, tol 1e-10, at most 500
+//              iterations; output int64 n, b, x, iterations, success, residual)
 //   op    : mass | adv | advface (face-centric operator on the interior faces)
 //           | intermesh_<F>_<PP> (3D, GLL: face F of an n^3 minus mesh of order
 //           P glued to an n^3 plus mesh of order PP, q shared; vector [minus|plus])
@@ -200,7 +205,7 @@ int main( int argc, char ** argv )
    a.coef = argv[9];
    a.bc = argv[10];
    a.seed = (unsigned)std::strtoul( argv[11], nullptr, 10 );
-   if ( a.mode == "fixture" || a.mode == "bicgstab" ) a.out = argv[12];
+   if ( a.mode == "fixture" || a.mode == "bicgstab" || a.mode == "gmres" || a.mode == "cg" ) a.out = argv[12];
    else a.pairs = std::atoi( argv[12] );
    if ( a.op == "adv6dprod" ) return run_product( a );
    if ( a.op.rfind( "padapt_", 0 ) == 0 ) return run_padapt_case( a );

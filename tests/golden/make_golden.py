@@ -75,14 +75,19 @@ def main(only=None):
             digest = hashlib.sha256(x.tobytes()).hexdigest()
             np.savez_compressed(HERE / f"{c.name}.npz", y=y, x_sha256=np.array(digest), n=np.array(x.size))
             print(f"{c.name}: {x.size} dofs, |y|={np.linalg.norm(y):.6e}")
-        # GenDiL's BiCGSTAB_no_alloc on M + 0.05 A (3D, n=4, GLL p=1, 2 points,
-        # C2 field, zero inflow), b = FillRandom(7), x0 = 0, tol 1e-10
-        if not wanted("bicgstab3d_c2_p1q2_bc0"):
-            return
-        b, x, it, ok, rel = oracle.ref_bicgstab(3, 1, 2, "L", 4, "c2", "zero", 7, Path(tmp) / "bicgstab.bin")
-        np.savez_compressed(HERE / "bicgstab3d_c2_p1q2_bc0.npz", x=x, iterations=np.array(it), success=np.array(ok),
-                            rel_residual=np.array(rel), b_sha256=np.array(hashlib.sha256(b.tobytes()).hexdigest()))
-        print(f"bicgstab3d_c2_p1q2_bc0: {b.size} dofs, {it} iterations, success {ok}, rel {rel:.3e}")
+        # GenDiL's own solvers: BiCGSTAB_no_alloc and GMRES_no_alloc(30) on
+        # M + 0.05 A (3D, n=4, GLL p=1, 2 points, C2 field, zero inflow),
+        # b = FillRandom(7); ConjugateGradient on M(1 + xy + z^2) (3D, n=5,
+        # GLL p=2, 3 points), b = FillRandom(5); x0 = 0, tol 1e-10
+        for name, args in (("bicgstab3d_c2_p1q2_bc0", ("bicgstab", 3, 1, 2, "L", 4, "c2", "zero", 7)),
+                           ("gmres3d_c2_p1q2_bc0", ("gmres", 3, 1, 2, "L", 4, "c2", "zero", 7)),
+                           ("cg3d_mass_p2q3_poly", ("cg", 3, 2, 3, "L", 5, "poly", "none", 5))):
+            if not wanted(name):
+                continue
+            b, x, it, ok, rel = oracle.ref_solve(*args, Path(tmp) / f"{name}.bin")
+            np.savez_compressed(HERE / f"{name}.npz", x=x, iterations=np.array(it), success=np.array(ok),
+                                rel_residual=np.array(rel), b_sha256=np.array(hashlib.sha256(b.tobytes()).hexdigest()))
+            print(f"{name}: {b.size} dofs, {it} iterations, success {ok}, rel {rel:.3e}")
 
 
 if __name__ == "__main__":

@@ -307,3 +307,63 @@ def test_solves_on_a_side_stream(solver):
     rel = np.linalg.norm(bb - Ko.apply(x.cpu().numpy())) / np.linalg.norm(bb)
     print(f"{solver} on a side stream: {res.iterations} iterations, oracle residual {rel:.3e}")
     assert rel <= 2e-10
+
+
+def test_gmres_matches_gendil_fixture():
+    # GenDiL's GMRES_no_alloc(30) on M + 0.05 A (p=1, 4^3, zero inflow),
+    # b = FillRandom(7), tol 1e-10: the device-resident GMRES takes the same
+    # number of iterations (within one) and lands on the same iterate
+    g = np.load(f"{oracle.HERE.parent}/tests/golden/gmres3d_c2_p1q2_bc0.npz")
+    mesh = G.CartesianMesh.unit(3, 4, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(1))
+    K = G.MakeMassAdvectionOperator(fes, G.MakeIntegrationRule(2), W.advection_c2(), bc=0.0, mass_coef=1.0,
+                                    adv_coef=0.05)
+    b = torch.as_tensor(G.fill_random(K.num_dofs, 7), device="cuda")
+    x = torch.zeros_like(b)
+    res = S.gmres(K, b, x, restart=30, max_iters=500, rel_tol=1e-10)
+    print(res, "reference iterations", int(g["iterations"]))
+    assert res.success
+    assert abs(res.iterations - int(g["iterations"])) <= 1
+    assert rel_l2(x.cpu().numpy(), g["x"]) <= 1e-8
+
+
+def test_cg_matches_gendil_fixture():
+    # GenDiL's ConjugateGradient on M(1 + xy + z^2) (p=2, 5^3), b =
+    # FillRandom(5), tol 1e-10: same iteration count within one, same iterate
+    from cases import sigma_field
+
+    g = np.load(f"{oracle.HERE.parent}/tests/golden/cg3d_mass_p2q3_poly.npz")
+    mesh = G.CartesianMesh.unit(3, 5, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(2))
+    M = G.MakeMassFiniteElementOperator(fes, G.MakeIntegrationRule(3), sigma_field("poly", 3))
+    b = torch.as_tensor(G.fill_random(M.num_dofs, 5), device="cuda")
+    x = torch.zeros_like(b)
+    res = S.cg(M, b, x, max_iters=500, rel_tol=1e-10)
+    print(res, "reference iterations", int(g["iterations"]))
+    assert res.success
+    assert abs(res.iterations - int(g["iterations"])) <= 1
+    assert rel_l2(x.cpu().numpy(), g["x"]) <= 1e-8
+
+
+@pytest.mark.parametrize("n,p", [(5, 2), (16, 3)])
+def test_cg_mixed_reaches_the_fp64_tolerance(n, p):
+    # FP32 inner CG refined in FP64 on the true residual: reaches 1e-10 on the
+    # mass matrix and lands on the FP64 CG's solution; checked with the oracle
+    from cases import sigma_field
+
+    mesh = G.CartesianMesh.unit(3, n, False)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(p))
+    ir = G.MakeIntegrationRule(p + 1)
+    M = G.MakeMassFiniteElementOperator(fes, ir, sigma_field("poly", 3))
+    bn = G.fill_random(M.num_dofs, 5)
+    b = torch.as_tensor(bn, device="cuda")
+    x = torch.zeros_like(b)
+    res = S.cg_mixed(M, b, x, rel_tol=1e-10)
+    print(res, flush=True)
+    assert res.success and res.final_rel_residual <= 1e-10 and res.refinements >= 2
+    P = oracle.Problem(fes, ir, mass_coef=1.0, sigma=sigma_field("poly", 3))
+    xn = x.cpu().numpy()
+    assert np.linalg.norm(bn - P.apply(xn)) / np.linalg.norm(bn) <= 2e-10
+    x64 = torch.zeros_like(b)
+    assert S.cg(M, b, x64, rel_tol=1e-10).success
+    assert rel_l2(xn, x64.cpu().numpy()) <= 1e-8
