@@ -58,7 +58,8 @@ int check_adv(const gd_plan* plan, bool need_pointwise)
 // with that many lanes per element; 16 + lanes = Kronecker-factored kernel
 // (lanes 1, 2, 4: register-resident; 32..39: LDS-staged, bit 0 = outputs
 // staged through LDS, bit 1 = 128-thread blocks, bit 2 = one term at a time;
-// 40: streaming).  Default: the streaming Kronecker kernel where it applies
+// 40, 41: streaming with two or one tile buffers).  Default: the streaming
+// Kronecker kernel with two buffers where it applies
 // (n = 4, rows a multiple of 16 elements; measured on MI355X at 64^3 p=3:
 // FP64 83 us against 97 us for the staged kernel, FP32 38 against 45 us),
 // else the staged Kronecker kernel, else the register Kronecker kernel, else
@@ -220,7 +221,7 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   if (plan->padapt)
     for (int v = 0; v < gd::pa::num_variants(plan->dim); ++v) choices.push_back(v);
   else
-    choices = {0, 1, 2, 4, 17, 18, 20, 48, 49, 50, 51, 52, 53, 54, 55, 56};
+    choices = {0, 1, 2, 4, 17, 18, 20, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57};
   const int saved = *knob;
   double best = 1e300;
   int best_lanes = 0;
@@ -301,7 +302,7 @@ int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes)
     return GD_OK;
   }
   if (lanes != -1 && lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 17 && lanes != 18 &&
-      lanes != 20 && !(lanes >= 48 && lanes <= 56))
+      lanes != 20 && !(lanes >= 48 && lanes <= 57))
     return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
   plan->tuned_k3_lanes[slot] = lanes;
   return GD_OK;

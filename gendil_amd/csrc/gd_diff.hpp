@@ -53,7 +53,10 @@ struct DCfg {
   static constexpr int FG = pick_fg();
   static constexpr int FACE_SCR = 2 * TPF * FG * MF;
   static constexpr int SCR = cmax(CELL_SCR, FACE_SCR);
-  static constexpr int PER_ELEM = 2 * ND + SCR;
+  // the diffusivity's 1D factor samples of the element (Q rule points, then
+  // xi = 0, xi = 1) per term and axis, 1 where a term has no factor
+  static constexpr int MUS = kMaxTerms * D * (Q + 2);
+  static constexpr int PER_ELEM = 2 * ND + SCR + MUS;
   static constexpr int LDS_BUDGET = 32 * 1024;
   static constexpr int EPB_RAW = LDS_BUDGET / (int)(PER_ELEM * sizeof(T));
   static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
@@ -61,7 +64,8 @@ struct DCfg {
 };
 
 template <class C>
-__global__ void __launch_bounds__(C::NT) diff_kernel(Params<typename C::T> p, const typename C::T* __restrict__ x,
+__global__ void __launch_bounds__(C::NT) diff_kernel(Params<typename C::T> p, FieldDev<typename C::T> mu,
+                                                     const typename C::T* __restrict__ x,
                                                      typename C::T* __restrict__ y)
 {
   using T = typename C::T;
@@ -92,6 +96,33 @@ __global__ void __launch_bounds__(C::NT) diff_kernel(Params<typename C::T> p, co
   auto Y = [&](int el) { return lds + el * PE + ND; };
   auto S = [&](int el, int k) { return lds + el * PE + 2 * ND + k * MXD; };
   auto R = [&](int el, int reg) { return lds + el * PE + 2 * ND + reg * (TPF * FG * MF); };
+  auto MU = [&](int el) { return lds + el * PE + 2 * ND + C::SCR; };
+
+  // the element's diffusivity factor samples, (term, axis) runs of Q + 2
+  constexpr int QS = Q + 2;
+  for (int i = threadIdx.x; i < EPB * mu.rank * D * QS; i += NT) {
+    const int el = i / (mu.rank * D * QS);
+    const int rr = i - el * (mu.rank * D * QS);
+    const int rd = rr / QS, k = rr - rd * QS;
+    const int r = rd / D, d = rd - r * D;
+    const T* tb = mu.tab[r][d];
+    T v = T(1);
+    if (tb && info[el].e >= 0) v = tb[info[el].sub[d] * QS + k];
+    MU(el)[rd * QS + k] = v;
+  }
+  __syncthreads();
+  // mu at per-axis sample indices k (k[d] < Q rule point, Q: xi = 0, Q + 1: xi = 1)
+  auto mu_at = [&](int el, const int* k) {
+    const T* m = MU(el);
+    T sum = T(0);
+    for (int r = 0; r < mu.rank; ++r) {
+      T t = mu.scale[r];
+#pragma unroll
+      for (int d = 0; d < D; ++d) t *= m[(r * D + d) * QS + k[d]];
+      sum += t;
+    }
+    return sum;
+  };
 
   const T* __restrict__ Bt = p.tab.B;
   const T* __restrict__ Gq = p.tab.Gq;
@@ -185,7 +216,7 @@ __global__ void __launch_bounds__(C::NT) diff_kernel(Params<typename C::T> p, co
         T w = T(1);
 #pragma unroll
         for (int d = 0; d < D; ++d) w *= (d == a) ? T(1) : wq[k[d]];
-        const T m = eval_sep<T, D, Q>(*p.mu, inf.sub, k);
+        const T m = mu_at(el, k);
         const T ov = tr[(0 * FG + j) * MF + t];
         const T g_own = tr[(1 * FG + j) * MF + t] * p.invh[a];
         T avg, jump, half;
@@ -273,28 +304,50 @@ __global__ void __launch_bounds__(C::NT) diff_kernel(Params<typename C::T> p, co
     T w = T(1);
 #pragma unroll
     for (int d = 0; d < D; ++d) w *= wq[k[d]];
-    S(el, CW)[qi] = info[el].e >= 0 ? w * p.detJ * eval_sep<T, D, Q>(*p.mu, info[el].sub, k) : T(0);
+    S(el, CW)[qi] = info[el].e >= 0 ? w * p.detJ * mu_at(el, k) : T(0);
   }
+  __syncthreads();  // every pencil below reads w detJ mu at points other threads wrote
   static_for<0, D>([&](auto d_) {
     constexpr int d = decltype(d_)::value;
     constexpr int INNER = Pow<Q, d>::v;
     constexpr int OUTER = Pow<Q, D - 1 - d>::v;
-    // FREE = Gq_d B u (the reference gradient along d at the rule points)
-    sweep<NT, EPB, OUTER, Q, Q, INNER, false, T>([&](int b) { return S(b, BU); }, [&](int b) { return S(b, FREE); },
-                                                [&](int j, int i) { return Gq[j * Q + i]; });
-    __syncthreads();
-    // inv_J applied twice (ApplyMapping, ApplyMappingTranspose), then w detJ mu
-    for (int i = threadIdx.x; i < EPB * QD; i += NT) {
-      const int el = i / QD, qi = i - el * QD;
-      T gv = S(el, FREE)[qi] * p.invh[d];
-      gv *= p.invh[d];
-      S(el, FREE)[qi] = S(el, CW)[qi] * gv;
+    constexpr int PEN = OUTER * INNER;
+    // G_d^T diag(w detJ mu / h_d^2) G_d acts within each pencil along d: one
+    // thread per pencil forms the reference gradient at the pencil's rule
+    // points (Gq_d B u), maps it with inv_J twice (ApplyMapping,
+    // ApplyMappingTranspose), scales it by w detJ mu and adds Gq_d^T of it
+    // into the accumulator
+    const T ih = p.invh[d];
+    for (int t = threadIdx.x; t < EPB * PEN; t += NT) {
+      const int b = t / PEN;
+      const int r = t - b * PEN;
+      const int o = r / INNER;
+      const int a = r - o * INNER;
+      const int base = o * Q * INNER + a;
+      const T* src = S(b, BU) + base;
+      const T* cw = S(b, CW) + base;
+      T* acc = S(b, ACCS) + base;
+      T v[Q], g[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) v[i] = src[i * INNER];
+#pragma unroll
+      for (int j = 0; j < Q; ++j) {
+        T s = T(0);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) s += Gq[j * Q + i] * v[i];
+        s *= ih;
+        s *= ih;
+        g[j] = cw[j * INNER] * s;
+      }
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < Q; ++j) s += Gq[j * Q + i] * g[j];
+        if constexpr (d == 0) acc[i * INNER] = s;
+        else acc[i * INNER] += s;
+      }
     }
-    __syncthreads();
-    // ACCS (+)= Gq_d^T FREE
-    sweep<NT, EPB, OUTER, Q, Q, INNER, (d > 0), T>([&](int b) { return S(b, FREE); },
-                                                 [&](int b) { return S(b, ACCS); },
-                                                 [&](int j, int i) { return Gq[i * Q + j]; });
     __syncthreads();
   });
   // ---- test functions: D sweeps Q -> N, first dim first, last into Y ----
@@ -331,8 +384,8 @@ constexpr size_t diff_lds_bytes()
 // Registry of compiled (dim, nodes, points) instances of the diffusion kernel.
 struct DiffSet {
   int D, N, Q;
-  int (*f64)(const Params<double>&, const double*, double*, hipStream_t);
-  int (*f32)(const Params<float>&, const float*, float*, hipStream_t);
+  int (*f64)(const Params<double>&, const FieldDev<double>*, const double*, double*, hipStream_t);
+  int (*f32)(const Params<float>&, const FieldDev<float>*, const float*, float*, hipStream_t);
 };
 
 struct DiffList {
@@ -345,7 +398,8 @@ DiffList diff_kernels_d3();
 DiffList diff_kernels_d6();
 
 template <class C>
-int launch_diff(const Params<typename C::T>& prm_in, const typename C::T* x, typename C::T* y, hipStream_t s)
+int launch_diff(const Params<typename C::T>& prm_in, const FieldDev<typename C::T>* mu, const typename C::T* x,
+                typename C::T* y, hipStream_t s)
 {
   using T = typename C::T;
   auto kern = dev::diff_kernel<C>;
@@ -363,7 +417,7 @@ int launch_diff(const Params<typename C::T>& prm_in, const typename C::T* x, typ
   if (nb > 0x7fffffffLL) return fail(GD_ERR_INVALID, "too many elements for one launch");
   prm.num_blocks = (int)nb;
   if (nb == 0) return GD_OK;
-  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(C::NT), lds, s, prm, x, y);
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(C::NT), lds, s, prm, *mu, x, y);
   GD_HIP(hipGetLastError());
   return GD_OK;
 }

@@ -40,8 +40,10 @@ int diff_apply(const gd_plan* plan, const T* x, T* y, hipStream_t s)
   const TableSet* ts = find_tables(plan, N, Q);
   if (!ts) return fail(GD_ERR_INVALID, "plan has no tables for its own rule");
   const Params<T> prm = make_params<T>(plan, ts, T(0), T(0));
-  if constexpr (std::is_same_v<T, double>) return ks->f64(prm, x, y, s);
-  else return ks->f32(prm, x, y, s);
+  // the diffusivity descriptor goes to the kernel by value (host copy of the
+  // device descriptor that prm.mu points to)
+  if constexpr (std::is_same_v<T, double>) return ks->f64(prm, &plan->fields64_host[plan->dim + 2], x, y, s);
+  else return ks->f32(prm, &plan->fields32_host[plan->dim + 2], x, y, s);
 }
 
 }  // namespace

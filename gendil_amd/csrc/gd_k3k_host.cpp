@@ -171,7 +171,7 @@ bool prepare(const gd_plan* plan, int lanes, bool adv, bool mass, KParams<T>& pr
   if (plan->dim != 3 || plan->padapt || plan->desc.basis != GD_BASIS_GLL || plan->desc.num_quad != N) return false;
   if (plan->E >= (int64_t)1 << 31) return false;
   // the streaming kernel deals tiles of 64 / N elements of one x-row
-  if (lanes == 40 && (64 % N != 0 || plan->desc.sizes[0] % (64 / N) != 0)) return false;
+  if ((lanes == 40 || lanes == 41) && (64 % N != 0 || plan->desc.sizes[0] % (64 / N) != 0)) return false;
   double sigma = 0.0;
   if (mass && !constant_of(plan->sigma_host, &sigma)) return false;
   int sigc = ksig(-1, -1, -1);

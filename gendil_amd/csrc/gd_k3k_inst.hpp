@@ -40,9 +40,10 @@ int launch_staged(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
   return GD_OK;
 }
 
-// Streaming kernel (lanes code 40): one wave per block, persistent over
-// tiles of one x-row, as many waves as fit on the device at once.
-template <typename T, int N, bool ADV, bool MASS, int SIG>
+// Streaming kernel (lanes code 40: two tile buffers per wave; 41: one):
+// one wave per block, persistent over tiles of one x-row, as many waves as
+// fit on the device at once.
+template <typename T, int N, bool ADV, bool MASS, int SIG, int NB>
 int launch_stream(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
 {
   using G = StreamGeom<T, N, ADV, SIG>;
@@ -55,13 +56,14 @@ int launch_stream(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
     GD_HIP(hipGetDevice(&dev));
     GD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     GD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k3k_stream<T, N, ADV, MASS, SIG>), 64, 0));
+        &per_cu, reinterpret_cast<const void*>(k3k_stream<T, N, ADV, MASS, SIG, NB>), 64, 0));
     waves = cus * (per_cu > 0 ? per_cu : 1);
   }
   KParams<T> prm = prm_in;
   const int grid = ntiles < waves ? ntiles : waves;
   prm.num_blocks = grid;
-  hipLaunchKernelGGL((k3k_stream<T, N, ADV, MASS, SIG>), dim3((unsigned)grid), dim3(64), 0, s, prm, x, y, ntiles);
+  hipLaunchKernelGGL((k3k_stream<T, N, ADV, MASS, SIG, NB>), dim3((unsigned)grid), dim3(64), 0, s, prm, x, y,
+                     ntiles);
   GD_HIP(hipGetLastError());
   return GD_OK;
 }
@@ -84,12 +86,12 @@ constexpr int kSigConst = ksig(-1, -1, -1);
   GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 0, kSigCyclic), GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 1, kSigCyclic),        \
       GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 0, kSigConst), GD_K3K_STAGED_ENTRY(T, N, CODE, 1, 1, kSigConst),      \
       GD_K3K_STAGED_ENTRY(T, N, CODE, 0, 1, kSigConst)
-#define GD_K3K_STREAM_ENTRY(T, N, ADV, MASS, SIG) \
-  KEntry<T> { N, 40, ADV, MASS, SIG, &launch_stream<T, N, ADV, MASS, SIG> }
-#define GD_K3K_STREAM(T, N)                                                                                  \
-  GD_K3K_STREAM_ENTRY(T, N, 1, 0, kSigCyclic), GD_K3K_STREAM_ENTRY(T, N, 1, 1, kSigCyclic),                   \
-      GD_K3K_STREAM_ENTRY(T, N, 1, 0, kSigConst), GD_K3K_STREAM_ENTRY(T, N, 1, 1, kSigConst),                 \
-      GD_K3K_STREAM_ENTRY(T, N, 0, 1, kSigConst)
+#define GD_K3K_STREAM_ENTRY(T, N, NB, ADV, MASS, SIG) \
+  KEntry<T> { N, 39 + NB, ADV, MASS, SIG, &launch_stream<T, N, ADV, MASS, SIG, NB> }
+#define GD_K3K_STREAM(T, N, NB)                                                                                \
+  GD_K3K_STREAM_ENTRY(T, N, NB, 1, 0, kSigCyclic), GD_K3K_STREAM_ENTRY(T, N, NB, 1, 1, kSigCyclic),             \
+      GD_K3K_STREAM_ENTRY(T, N, NB, 1, 0, kSigConst), GD_K3K_STREAM_ENTRY(T, N, NB, 1, 1, kSigConst),           \
+      GD_K3K_STREAM_ENTRY(T, N, NB, 0, 1, kSigConst)
 #define GD_K3K_STAGED_ALL(T, N)                                                                        \
   GD_K3K_STAGED(T, N, 0), GD_K3K_STAGED(T, N, 1), GD_K3K_STAGED(T, N, 2), GD_K3K_STAGED(T, N, 3),          \
       GD_K3K_STAGED(T, N, 4), GD_K3K_STAGED(T, N, 5), GD_K3K_STAGED(T, N, 6), GD_K3K_STAGED(T, N, 7)

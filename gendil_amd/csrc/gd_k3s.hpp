@@ -15,9 +15,10 @@
 //   fall on a pad chunk or past the tile copy an in-bounds chunk that is
 //   never read.
 // * Wave-local: no block barrier at all.  The wave waits for a tile's copies
-//   with s_waitcnt vmcnt(INS) once the next tile's INS copies are issued
-//   (vector memory operations complete in issue order), so those stay in
-//   flight; no other vector load is issued in the loop.
+//   with s_waitcnt vmcnt(INS + ST) once the next tile's INS copies are issued
+//   (vector memory operations complete in issue order), so those and the
+//   previous tile's ST output stores stay in flight; no other vector memory
+//   operation is issued in the loop.
 // * Outputs are staged through the tile's element region and leave as
 //   coalesced 16-byte non-temporal stores.
 // Tiles are dealt to waves so that the waves of one XCD (blocks go round-robin
@@ -49,6 +50,22 @@ __device__ __forceinline__ void dma16(const void* src, unsigned lds)
   __asm__ volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" : : "v"(src), "s"(lds) : "memory", "m0");
 }
 #pragma clang diagnostic pop
+
+// Lane K of this lane's quad (lanes 4i .. 4i+3): a DPP quad_perm broadcast,
+// no LDS traffic
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), K * 0x55, 0xf, 0xf, false));
+}
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v)
+{
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, K * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), K * 0x55, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
 
 // s_waitcnt vmcnt(K), the other counters left alone (gfx9 encoding: vmcnt
 // bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8), then a compiler fence so no
@@ -96,8 +113,11 @@ struct StreamGeom {
   static constexpr int ZT0 = NRUN * TW * SCS;   // first chunk of the z-factor table
   static constexpr int S_INS = (ZT0 + NTZ * SCH + 63) / 64;
   static constexpr int INS = U_INS + S_INS;     // copies per tile and lane
+  static constexpr int ST = TW * CPE / 64;      // output stores per tile and lane
   static constexpr int BUF_CHUNKS = INS * 64;
   static_assert(N3 % VW == 0 && N2 % VW == 0, "element planes must be whole 16-byte chunks");
+  static_assert((TW * CPE) % 64 == 0, "a tile's outputs are whole wave-wide stores");
+  static_assert(INS + ST < 64, "vmcnt counts to 63");
   static_assert(NTZ <= 1, "at most one component depends on z");
   static_assert(!(ADV && (AX[0] == 0 || AX[1] == 1 || AX[2] == 2)),
                 "components varying along their own axis read face samples from memory: not served here");
@@ -191,9 +211,9 @@ __device__ __forceinline__ void stream_issue(const int (&kind)[G::INS], const in
 // Elements are dealt in tiles of TW consecutive elements of one x-row; the
 // host requires sizes[0] to be a multiple of TW (so every tile is full) and
 // no component varying along its own axis.
-template <typename T, int N, bool ADV, bool MASS, int SIG>
-__global__ void __launch_bounds__(64) k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y,
-                                                 int ntiles)
+template <typename T, int N, bool ADV, bool MASS, int SIG, int NB>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 2 : 1)))
+k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
 {
   using G = StreamGeom<T, N, ADV, SIG>;
   constexpr int N2 = G::N2, N3 = G::N3, TW = G::TW, VW = G::VW, CPE = G::CPE, UCS = G::UCS, SCS = G::SCS;
@@ -202,7 +222,8 @@ __global__ void __launch_bounds__(64) k3k_stream(KParams<T> p, const T* __restri
   constexpr bool MSH = ADV && MASS && G::AX[0] != 2;
   using KT = KTab<N>;
   using C = Chunk<T, VW>;
-  __shared__ C lds[2 * G::BUF_CHUNKS];
+  static_assert(NB == 1 || NB == 2, "one or two tile buffers");
+  __shared__ C lds[NB * G::BUF_CHUNKS];
   const unsigned lds_base = (unsigned)(size_t)(__attribute__((address_space(3))) void*)lds;
 
   const int lane = (int)threadIdx.x;
@@ -291,7 +312,7 @@ __global__ void __launch_bounds__(64) k3k_stream(KParams<T> p, const T* __restri
   };
 
   // ---- one tile: compute from buffer b while the next tile moves into b^1 ----
-  auto step = [&](int t, int b) __attribute__((always_inline)) {
+  auto step = [&](int t, int b, bool first) __attribute__((always_inline)) {
     C* cur = lds + b * G::BUF_CHUNKS;
     const int tn = t + nw;
     const TilePos tp = tile_pos(t);
@@ -323,11 +344,23 @@ __global__ void __launch_bounds__(64) k3k_stream(KParams<T> p, const T* __restri
       for (int d = 0; d < 3; ++d) fc.cown[d] = fc.cnb[d] = T(0);
     }
 
-    if (tn < ntiles) {
-      issue(tn, b ^ 1);
-      wait_vmcnt<G::INS>();
+    // In flight, oldest first: this tile's copies, the previous tile's output
+    // stores (none for a wave's first tile), the next tile's copies.  Wait for
+    // this tile's copies only: the stores and the next copies stay in flight.
+    if constexpr (NB == 2) {
+      if (tn < ntiles) {
+        issue(tn, b ^ 1);
+        if (first) wait_vmcnt<G::INS>();
+        else wait_vmcnt<G::INS + G::ST>();
+      } else {
+        if (first) wait_vmcnt<0>();
+        else wait_vmcnt<G::ST>();
+      }
     } else {
-      wait_vmcnt<0>();
+      // one buffer: in flight, oldest first, this tile's copies and the
+      // previous tile's stores (issued after them)
+      if (first) wait_vmcnt<0>();
+      else wait_vmcnt<G::ST>();
     }
 
     const C* ue = cur + el * UCS;
@@ -419,11 +452,24 @@ __global__ void __launch_bounds__(64) k3k_stream(KParams<T> p, const T* __restri
         T sx[N];
 #pragma unroll
         for (int j = 0; j < N; ++j) sx[j] = T(0);
+        if constexpr (N == 4) {
+          // the element's four lanes are one quad
+          const T c0 = coefX(0), c1 = coefX(1), c2 = coefX(2), c3 = coefX(3);
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-          const T ck = coefX(k);
+          for (int j = 0; j < N; ++j) sx[j] += c0 * quad_bcast<0>(vx[j]);
 #pragma unroll
-          for (int j = 0; j < N; ++j) sx[j] += ck * __shfl(vx[j], (lane & ~(N - 1)) + k, 64);
+          for (int j = 0; j < N; ++j) sx[j] += c1 * quad_bcast<1>(vx[j]);
+#pragma unroll
+          for (int j = 0; j < N; ++j) sx[j] += c2 * quad_bcast<2>(vx[j]);
+#pragma unroll
+          for (int j = 0; j < N; ++j) sx[j] += c3 * quad_bcast<3>(vx[j]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < N; ++k) {
+            const T ck = coefX(k);
+#pragma unroll
+            for (int j = 0; j < N; ++j) sx[j] += ck * __shfl(vx[j], (lane & ~(N - 1)) + k, 64);
+          }
         }
         term_x<T, N, SIG>(p, fc, ftab(0), accX, sx, out);
       }
@@ -465,18 +511,36 @@ __global__ void __launch_bounds__(64) k3k_stream(KParams<T> p, const T* __restri
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     C* dst = reinterpret_cast<C*>(y) + (long long)e0 * CPE;
+    if constexpr (NB == 2) {
 #pragma unroll
-    for (int i = 0; i < (TW * CPE) / 64; ++i) {
-      const int cidx = i * 64 + lane;
-      const int ee = cidx / CPE;
-      store_nt(&dst[cidx], cur[ee * UCS + (cidx - ee * CPE)]);
+      for (int i = 0; i < G::ST; ++i) {
+        const int cidx = i * 64 + lane;
+        const int ee = cidx / CPE;
+        store_nt(&dst[cidx], cur[ee * UCS + (cidx - ee * CPE)]);
+      }
+    } else {
+      // one buffer: take the staged outputs into registers, let every LDS
+      // read finish (lgkmcnt(0)), then refill the buffer with the next
+      // tile's copies before the stores go out
+      C st[G::ST];
+#pragma unroll
+      for (int i = 0; i < G::ST; ++i) {
+        const int cidx = i * 64 + lane;
+        const int ee = cidx / CPE;
+        st[i] = cur[ee * UCS + (cidx - ee * CPE)];
+      }
+      __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));
+      __builtin_amdgcn_wave_barrier();
+      if (tn < ntiles) issue(tn, 0);
+#pragma unroll
+      for (int i = 0; i < G::ST; ++i) store_nt(&dst[i * 64 + lane], st[i]);
     }
     __builtin_amdgcn_wave_barrier();
   };
 
   int t = w;
   if (t < ntiles) issue(t, 0);
-  for (int b = 0; t < ntiles; t += nw, b ^= 1) step(t, b);
+  for (int b = 0; t < ntiles; t += nw, b = NB == 2 ? b ^ 1 : 0) step(t, b, t == w);
 }
 
 }  // namespace k3k

@@ -52,8 +52,8 @@ def variants_served(op, dtype, mass_coef, adv_coef, candidates):
 
 # generic kernel, register kernel (1, 2, 4 lanes), register Kronecker kernel
 # (17, 18, 20), the LDS-staged Kronecker kernel (48..55) and its streaming
-# form (56)
-CANDIDATES = [0, 1, 2, 4, 17, 18, 20, 48, 49, 50, 51, 52, 53, 56]
+# forms (56, 57)
+CANDIDATES = [0, 1, 2, 4, 17, 18, 20, 48, 49, 50, 51, 52, 53, 56, 57]
 
 
 def check_variants(op, x, y_ref, mass_coef, adv_coef, label):

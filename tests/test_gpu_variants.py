@@ -20,7 +20,7 @@ ELIGIBLE = ["adv3d_c2_p3q4_per", "adv3d_c2_p3q4_bc0", "adv3d_c2_p3q4_nobc", "adv
             "adv3d_bubble_p2q4", "adv3d_bubble_p3q5"]
 STAGED = list(range(48, 56))
 KRON = {1: [17, 18] + STAGED, 2: [17], 3: [18, 20] + STAGED}
-ALL_VARIANTS = (0, 1, 2, 4, 17, 18, 20, *STAGED, 56)
+ALL_VARIANTS = (0, 1, 2, 4, 17, 18, 20, *STAGED, 56, 57)
 
 
 def variants(case):
@@ -44,7 +44,7 @@ def test_all_variants_match_golden(name, dtype):
         assert err <= (1e-12 if dtype == torch.float64 else 1e-5), (lanes, err)
 
 
-@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 18, 20, *STAGED, 56])
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 18, 20, *STAGED, 56, 57])
 def test_variants_combined_operator_at_16_cubed(lanes):
     mesh = G.CartesianMesh.unit(3, 16, False)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
@@ -197,9 +197,12 @@ STREAM_MESHES = [
 
 @pytest.mark.parametrize("mesh_case", STREAM_MESHES, ids=["x16y13", "y1", "x48", "tiles4096"])
 @pytest.mark.parametrize("kind", ["c2", "negc2", "const"])
-def test_streaming_kernel_vs_oracle(mesh_case, kind):
-    """Variant 56 (one wave per block, persistent over tiles of 16 elements of
-    one x-row, copies of the next tile in flight while a tile computes)."""
+@pytest.mark.parametrize("variant", [56, 57])
+def test_streaming_kernel_vs_oracle(mesh_case, kind, variant):
+    """Variants 56 and 57 (one wave per block, persistent over tiles of 16
+    elements of one x-row; 56: two tile buffers, the next tile's copies in
+    flight while a tile computes; 57: one buffer, refilled as the tile's
+    outputs leave, two waves per SIMD)."""
     sizes, periodic = mesh_case
     mesh = G.CartesianMesh(sizes, [1.0 / s for s in sizes], [0.0] * 3, periodic)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
@@ -210,9 +213,9 @@ def test_streaming_kernel_vs_oracle(mesh_case, kind):
         x = G.fill_random(op.num_dofs, 23)
         ref = oracle.Problem(fes, ir, mass_coef=mc, adv_coef=ac, adv=adv, bc=bc).apply(x)
         for dtype in (torch.float64, torch.float32):
-            print(f"stream {sizes} {kind} m={mc} a={ac} {dtype}", flush=True)
-            op.plan.set_variant(dtype, 56)
-            assert op.plan.query_variant(dtype, mc, ac) == 56
+            print(f"stream {variant} {sizes} {kind} m={mc} a={ac} {dtype}", flush=True)
+            op.plan.set_variant(dtype, variant)
+            assert op.plan.query_variant(dtype, mc, ac) == variant
             y = torch.full((op.num_dofs,), float("nan"), dtype=dtype, device="cuda")
             op(torch.as_tensor(x, dtype=dtype, device="cuda"), y)
             err = rel_l2(y.double().cpu().numpy(), ref)
@@ -224,8 +227,9 @@ def test_streaming_kernel_needs_whole_tiles():
     mesh = G.CartesianMesh([20, 4, 4], [0.05, 0.25, 0.25], [0.0] * 3, True)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
     op = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(4), W.advection_c2())
-    op.plan.set_variant(torch.float64, 56)
-    assert op.plan.query_variant(torch.float64, 0.0, 1.0) != 56
+    for v in (56, 57):
+        op.plan.set_variant(torch.float64, v)
+        assert op.plan.query_variant(torch.float64, 0.0, 1.0) != v
     x = G.fill_random(op.num_dofs, 2)
     y = torch.empty(op.num_dofs, dtype=torch.float64, device="cuda")
     op(torch.as_tensor(x, device="cuda"), y)
