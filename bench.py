@@ -54,7 +54,7 @@ def measured_traffic(key):
     """HBM bytes per launch of the default apply kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
     copy); None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k3k_staged_adv3d_64cubed_p3_pmc.md")
+    path = os.path.join(ROOT, "profiles", "k3k_stream_adv3d_64cubed_p3_pmc.md")
     try:
         with open(path) as f:
             for line in f:

@@ -58,10 +58,11 @@ int check_adv(const gd_plan* plan, bool need_pointwise)
 // with that many lanes per element; 16 + lanes = Kronecker-factored kernel
 // (lanes 1, 2, 4: register-resident; 32..39: LDS-staged, bit 0 = outputs
 // staged through LDS, bit 1 = 128-thread blocks, bit 2 = one term at a time;
-// 40, 41: streaming with two or one tile buffers).  Default: the streaming
-// Kronecker kernel with two buffers where it applies
+// 40, 41: streaming with one or two tile buffers).  Default: the streaming
+// Kronecker kernel with one buffer where it applies
 // (n = 4, rows a multiple of 16 elements; measured on MI355X at 64^3 p=3:
-// FP64 83 us against 97 us for the staged kernel, FP32 38 against 45 us),
+// FP64 67 us against 97 us for the staged kernel and 77-85 us with two
+// buffers, FP32 34 against 45 us),
 // else the staged Kronecker kernel, else the register Kronecker kernel, else
 // the register kernel.
 template <typename T>

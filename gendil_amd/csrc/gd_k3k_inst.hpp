@@ -40,7 +40,8 @@ int launch_staged(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
   return GD_OK;
 }
 
-// Streaming kernel (lanes code 40: two tile buffers per wave; 41: one):
+// Streaming kernel (lanes code 39 + NB: 40 one tile buffer per wave, two
+// waves per SIMD; 41 two buffers, one wave per SIMD):
 // one wave per block, persistent over tiles of one x-row, as many waves as
 // fit on the device at once.
 template <typename T, int N, bool ADV, bool MASS, int SIG, int NB>

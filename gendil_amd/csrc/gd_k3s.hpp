@@ -4,26 +4,29 @@
 //
 // * One wave per block, persistent: it walks a sequence of tiles of TW = 64/N
 //   consecutive elements of one x-row (N lanes per element, lane h producing
-//   output z-plane h) and keeps two LDS buffers.
-// * While tile t computes from one buffer, tile t+1's element chunks,
-//   upwind neighbour slabs and beta-dependent factor tables move into the
-//   other buffer by the gfx950 LDS DMA (global_load_lds_dwordx4: every lane
-//   sends one 16-byte chunk from its own global address to the buffer at
-//   base + 16 * lane), with no registers held and no LDS stores executed by
-//   the wave.  Each element occupies CPE + 1 chunks (one pad chunk, so the 16
-//   elements a wave reads at one offset spread over the banks); lanes that
-//   fall on a pad chunk or past the tile copy an in-bounds chunk that is
-//   never read.
+//   output z-plane h).
+// * A tile's element chunks, upwind neighbour slabs and beta-dependent factor
+//   tables move into LDS by the gfx950 LDS DMA (global_load_lds_dwordx4:
+//   every lane sends one 16-byte chunk from its own global address to the
+//   buffer at base + 16 * lane), with no registers held and no LDS stores
+//   executed by the wave.  Each element occupies CPE + 1 chunks (one pad
+//   chunk, so the 16 elements a wave reads at one offset spread over the
+//   banks); lanes that fall on a pad chunk or past the tile copy an in-bounds
+//   chunk that is never read.
+// * NB = 1 (two waves per SIMD): one buffer, refilled with the next tile's
+//   copies as soon as the staged outputs are in registers; the stores follow
+//   the copies.  NB = 2 (one wave per SIMD): while tile t computes from one
+//   buffer, tile t+1's copies fill the other.
 // * Wave-local: no block barrier at all.  The wave waits for a tile's copies
-//   with s_waitcnt vmcnt(INS + ST) once the next tile's INS copies are issued
-//   (vector memory operations complete in issue order), so those and the
-//   previous tile's ST output stores stay in flight; no other vector memory
-//   operation is issued in the loop.
+//   with s_waitcnt vmcnt (vector memory operations complete in issue order),
+//   leaving the previous tile's output stores (and, with two buffers, the
+//   next tile's copies) in flight; no other vector memory operation is
+//   issued in the loop.
 // * Outputs are staged through the tile's element region and leave as
 //   coalesced 16-byte non-temporal stores.
-// Tiles are dealt to waves so that the waves of one XCD (blocks go round-robin
-// over the 8 XCDs) take consecutive tiles at any moment, which keeps the y
-// neighbours of a tile in that XCD's L2.
+// Each XCD walks one contiguous chunk of tiles (TileWalk), so the y and z
+// neighbours of a tile were read by the same XCD shortly before and are
+// still in its L2.
 #pragma once
 
 #include <array>
@@ -85,10 +88,26 @@ struct StreamGeom {
   static constexpr int VW = 16 / (int)sizeof(T);
   static constexpr int CPE = N3 / VW;           // chunks per element
   static constexpr int UCS = CPE + 1;           // element stride in chunks
+  static constexpr int PC = N2 / VW;            // chunks per z-plane of an element
+  // Element chunk c (z-plane c / PC) sits at position swz(c) of its LDS
+  // slot: the chunk index within the plane is XORed with a function of the
+  // plane, so the N lanes of an element, which write N different planes at
+  // one chunk index, and the lanes reading the x-neighbour slab (one value of
+  // every plane) fall on different banks.  XOR within a plane: an
+  // involution, swz(swz(c)) = c.
+  static constexpr int PX(int plane)
+  {
+    if constexpr (N != 4) return 0;
+    else return VW == 2 ? 2 * plane : 2 * (plane >> 1);
+  }
+  static constexpr int swz(int c) { return c ^ PX(c / PC); }
+  // the same XOR in units of x-rows (N values): plane p's row s holds row s ^ QX(p)
+  static constexpr int QX(int plane) { return PX(plane) * VW / N; }
+  static_assert(N != 4 || (PX(1) * VW) % N == 0, "the swizzle moves whole x-rows");
   // the tile's elements, then one halo slot: the upwind x neighbour of the
   // tile's first (beta_x >= 0) or last (beta_x <= 0) element
   static constexpr int U_INS = ((TW + 1) * UCS + 63) / 64;
-  static constexpr int SCH = N2 / VW;           // chunks per slab / table
+  static constexpr int SCH = PC;                // chunks per slab / table
   static constexpr int SCS = SCH + 1;
   static constexpr int AX[3] = {ksig_axis(SIG, 0), ksig_axis(SIG, 1), ksig_axis(SIG, 2)};
   static constexpr bool TIN[3] = {ADV && (AX[0] == 0 || AX[0] == 1), ADV && (AX[1] == 0 || AX[1] == 1),
@@ -140,8 +159,8 @@ __host__ __device__ constexpr StreamSrc stream_src(int L, bool second)
   if (!second) {
     const int ee = L / UCS, c = L - ee * UCS;
     if (c >= CPE || ee > TW) return {8, 0};
-    if (ee == TW) return {1, c * VW};
-    return {0, ee * N3 + c * VW};
+    if (ee == TW) return {1, G::swz(c) * VW};
+    return {0, ee * N3 + G::swz(c) * VW};
   }
   if (L >= G::ZT0) {
     const int zc = L - G::ZT0;
@@ -208,6 +227,34 @@ __device__ __forceinline__ void stream_issue(const int (&kind)[G::INS], const in
   }
 }
 
+// The tiles one wave walks.  XCD x (blocks go round-robin over the 8 XCDs)
+// owns one contiguous chunk of tiles, in proportion to its waves, and its wx
+// waves walk it wx consecutive tiles at a time, so the upwind z-neighbour
+// plane of a tile was read by the same XCD one plane earlier and is still in
+// its L2 (`rev`: the walk runs downwards, for beta_z <= 0).  Each XCD starts
+// at a staggered point of its chunk and wraps round, so the eight concurrent
+// streams do not sit a power-of-two distance apart in memory.  Position jj
+// of the chunk is tile(jj); a wave visits j0, next(j0), ... (count tiles).
+struct TileWalk {
+  int c0, len, wx, count, j0;
+  bool rev;
+  __host__ __device__ TileWalk(int block, int nw, int ntiles, bool down)
+  {
+    const int xcd = block % 8, r = block / 8;
+    const int per = nw / 8, rem = nw % 8;
+    wx = per + (xcd < rem ? 1 : 0);
+    const int w0 = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+    c0 = (int)((long long)ntiles * w0 / nw);
+    len = (int)((long long)ntiles * (w0 + wx) / nw) - c0;
+    count = r < len ? (len - r + wx - 1) / wx : 0;
+    j0 = count > 0 ? (int)((r + (long long)xcd * (len / 8 + 37)) % len) : 0;
+    rev = down;
+  }
+  __host__ __device__ int tile(int jj) const { return rev ? c0 + len - 1 - jj : c0 + jj; }
+  // a wave with a next tile has wx < len, so one wrap suffices
+  __host__ __device__ int next(int jj) const { return jj + wx >= len ? jj + wx - len : jj + wx; }
+};
+
 // Elements are dealt in tiles of TW consecutive elements of one x-row; the
 // host requires sizes[0] to be a multiple of TW (so every tile is full) and
 // no component varying along its own axis.
@@ -232,11 +279,8 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
   const bool negx = (p.negmask & 1) != 0, negy = (p.negmask & 2) != 0, negz = (p.negmask & 4) != 0;
   const int ny = negy ? 0 : N - 1, nz = negz ? 0 : N - 1;  // node of the upwind neighbour slab
 
-  // waves of one XCD take consecutive tiles: logical wave index
-  const int nw = (int)gridDim.x;
-  const int xcd = (int)blockIdx.x % 8, r = (int)blockIdx.x / 8;
-  const int per = nw / 8, rem = nw % 8;
-  const int w = xcd < rem ? xcd * (per + 1) + r : rem * (per + 1) + (xcd - rem) * per + r;
+  const TileWalk walk((int)blockIdx.x, (int)gridDim.x, ntiles, negz);
+  int j = walk.j0;
 
   // this lane's fixed copy offsets (values) and bases, one per instruction
   int kind[G::INS], off[G::INS];
@@ -312,9 +356,8 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
   };
 
   // ---- one tile: compute from buffer b while the next tile moves into b^1 ----
-  auto step = [&](int t, int b, bool first) __attribute__((always_inline)) {
+  auto step = [&](int t, int tn, int b, bool first) __attribute__((always_inline)) {
     C* cur = lds + b * G::BUF_CHUNKS;
-    const int tn = t + nw;
     const TilePos tp = tile_pos(t);
     const int e0 = tp.e0;
     // face coefficients: y and z faces are the same for the whole tile
@@ -348,7 +391,7 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
     // stores (none for a wave's first tile), the next tile's copies.  Wait for
     // this tile's copies only: the stores and the next copies stay in flight.
     if constexpr (NB == 2) {
-      if (tn < ntiles) {
+      if (tn >= 0) {
         issue(tn, b ^ 1);
         if (first) wait_vmcnt<G::INS>();
         else wait_vmcnt<G::INS + G::ST>();
@@ -384,7 +427,9 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
     if constexpr (ADV) {
       const int xs = negx ? el + 1 : el - 1;
       const int slot = (xs >= 0 && xs < TW) ? xs : TW;
-      const T* un = reinterpret_cast<const T*>(cur + slot * UCS) + h * N2 + fc.nsl[0];
+      // the neighbour's plane h, row s holds y-row s ^ QX(h) (the swizzle
+      // permutes whole rows of x values): vx[s] = value (x = nsl, y = s ^ QX(h), z = h)
+      const T* un = reinterpret_cast<const T*>(cur + slot * UCS + h * G::PC) + fc.nsl[0];
 #pragma unroll
       for (int j = 0; j < N; ++j) vx[j] = un[j * N];
     }
@@ -433,7 +478,7 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
         }
 #pragma unroll
         for (int v = 0; v < N2 / VW; ++v) {
-          const C c = ue[j * (N2 / VW) + v];
+          const C c = ue[G::swz(j * G::PC + v)];
 #pragma unroll
           for (int l = 0; l < VW; ++l) {
             const int m = v * VW + l;
@@ -456,13 +501,13 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
           // the element's four lanes are one quad
           const T c0 = coefX(0), c1 = coefX(1), c2 = coefX(2), c3 = coefX(3);
 #pragma unroll
-          for (int j = 0; j < N; ++j) sx[j] += c0 * quad_bcast<0>(vx[j]);
+          for (int j = 0; j < N; ++j) sx[j] += c0 * quad_bcast<0>(vx[j ^ G::QX(0)]);
 #pragma unroll
-          for (int j = 0; j < N; ++j) sx[j] += c1 * quad_bcast<1>(vx[j]);
+          for (int j = 0; j < N; ++j) sx[j] += c1 * quad_bcast<1>(vx[j ^ G::QX(1)]);
 #pragma unroll
-          for (int j = 0; j < N; ++j) sx[j] += c2 * quad_bcast<2>(vx[j]);
+          for (int j = 0; j < N; ++j) sx[j] += c2 * quad_bcast<2>(vx[j ^ G::QX(2)]);
 #pragma unroll
-          for (int j = 0; j < N; ++j) sx[j] += c3 * quad_bcast<3>(vx[j]);
+          for (int j = 0; j < N; ++j) sx[j] += c3 * quad_bcast<3>(vx[j ^ G::QX(3)]);
         } else {
 #pragma unroll
           for (int k = 0; k < N; ++k) {
@@ -500,12 +545,22 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
     // outputs through the tile's element region: every lane has finished
     // reading `cur` (the values were consumed), so the wave may overwrite it
     __builtin_amdgcn_wave_barrier();
+    {
+      // PX is even: chunks 2k and 2k+1 stay neighbours, at (2k) ^ px
+      static_assert(G::PC % 2 == 0, "pairs of chunks per plane");
+      C* ow = cur + el * UCS + h * G::PC;
+      const int px = G::PX(h);
 #pragma unroll
-    for (int v = 0; v < N2 / VW; ++v) {
-      C c;
+      for (int k = 0; k < G::PC / 2; ++k) {
+        C* w2 = ow + ((2 * k) ^ px);
 #pragma unroll
-      for (int l = 0; l < VW; ++l) c.v[l] = out[v * VW + l];
-      cur[el * UCS + h * (N2 / VW) + v] = c;
+        for (int i = 0; i < 2; ++i) {
+          C c;
+#pragma unroll
+          for (int l = 0; l < VW; ++l) c.v[l] = out[(2 * k + i) * VW + l];
+          w2[i] = c;
+        }
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -516,7 +571,7 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
       for (int i = 0; i < G::ST; ++i) {
         const int cidx = i * 64 + lane;
         const int ee = cidx / CPE;
-        store_nt(&dst[cidx], cur[ee * UCS + (cidx - ee * CPE)]);
+        store_nt(&dst[cidx], cur[ee * UCS + G::swz(cidx - ee * CPE)]);
       }
     } else {
       // one buffer: take the staged outputs into registers, let every LDS
@@ -527,20 +582,23 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
       for (int i = 0; i < G::ST; ++i) {
         const int cidx = i * 64 + lane;
         const int ee = cidx / CPE;
-        st[i] = cur[ee * UCS + (cidx - ee * CPE)];
+        st[i] = cur[ee * UCS + G::swz(cidx - ee * CPE)];
       }
       __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));
       __builtin_amdgcn_wave_barrier();
-      if (tn < ntiles) issue(tn, 0);
+      if (tn >= 0) issue(tn, 0);
 #pragma unroll
       for (int i = 0; i < G::ST; ++i) store_nt(&dst[i * 64 + lane], st[i]);
     }
     __builtin_amdgcn_wave_barrier();
   };
 
-  int t = w;
-  if (t < ntiles) issue(t, 0);
-  for (int b = 0; t < ntiles; t += nw, b = NB == 2 ? b ^ 1 : 0) step(t, b, t == w);
+  if (walk.count > 0) issue(walk.tile(j), 0);
+  for (int k = 0, b = 0; k < walk.count; ++k, b = NB == 2 ? b ^ 1 : 0) {
+    const int jn = walk.next(j);
+    step(walk.tile(j), k + 1 < walk.count ? walk.tile(jn) : -1, b, k == 0);
+    j = jn;
+  }
 }
 
 }  // namespace k3k

@@ -213,11 +213,12 @@ int gd_adv_hmesh_apply_f64(const gd_plan* coarse, const gd_plan* fine, int32_t f
  * the register-resident 3D kernel with 1, 2 or 4 lanes per element, and the
  * Kronecker-factored 3D kernel (16 + lanes per element: 17, 18, 20) and its
  * LDS-staged form (48..55: 48 + 1 outputs staged through LDS, + 2 blocks of
- * 128 threads, + 4 one term at a time) and its streaming form (56: one wave
- * per block, persistent over tiles of 16 elements of one x-row, the next
- * tile's copies into LDS in flight while a tile computes; 57: one buffer,
- * refilled as a tile's outputs leave, two waves per SIMD; rows must be a
- * multiple of 16 elements) where they apply.  p-adaptive plans:
+ * 128 threads, + 4 one term at a time) and its streaming form (one wave per
+ * block, persistent over tiles of 16 elements of one x-row; 56: one tile
+ * buffer, refilled as soon as a tile's outputs are in registers, two waves
+ * per SIMD; 57: two tile buffers, the next tile's copies into LDS in flight
+ * while a tile computes, one wave per SIMD; rows must be a multiple of 16
+ * elements) where they apply.  p-adaptive plans:
  * the thread-group layouts of the quadrature kernel (0 = one wave per n=3
  * element, 1 = one block per n=3 element; n=4 elements take a block, n=2
  * elements a wave) and, in 6D for a field with paired affine components of

@@ -4,7 +4,9 @@
 // its buffer (the input vector or a factor table) and is 16-byte aligned, and
 // each instruction's lanes take their bases from the kinds it was compiled
 // for.  The per-tile bases restate the kernel's `issue` lambda; the per-lane
-// sources are the kernel's own stream_src.  Runs on the CPU only.
+// sources are the kernel's own stream_src.  Also: the element region's chunk
+// layout (StreamGeom::swz) against those sources, and the kernel's TileWalk
+// visiting every tile once.  Runs on the CPU only.
 #include "gd_k3s.hpp"
 #include <cstdio>
 #include <cstdlib>
@@ -83,6 +85,63 @@ long check(int M0, int M1, int M2, int periodic, int negmask)
   return nbad;
 }
 
+// The element region of a tile buffer: LDS position ee * UCS + swz(c) of an
+// element slot receives chunk c of that element (and the halo slot chunk c of
+// the halo element), every chunk exactly once, as the kernel's reads assume.
+template <typename T, int N, bool ADV, int SIG>
+long check_layout()
+{
+  using G = StreamGeom<T, N, ADV, SIG>;
+  long nbad = 0;
+  for (int c = 0; c < G::CPE; ++c)
+    if (G::swz(G::swz(c)) != c || G::swz(c) / G::PC != c / G::PC) ++nbad;
+  for (int ee = 0; ee <= G::TW; ++ee) {
+    int seen[G::CPE] = {};
+    for (int p = 0; p < G::UCS; ++p) {
+      const StreamSrc s = stream_src<G>(ee * G::UCS + p, false);
+      if (p >= G::CPE) {
+        if (s.kind != 8) ++nbad;
+        continue;
+      }
+      const int want_kind = ee == G::TW ? 1 : 0;
+      const int base = ee == G::TW ? 0 : ee * G::N3;
+      if (s.kind != want_kind || (s.off - base) % G::VW) { ++nbad; continue; }
+      const int c = (s.off - base) / G::VW;
+      if (c < 0 || c >= G::CPE || G::swz(c) != p) { ++nbad; continue; }
+      ++seen[c];
+    }
+    for (int c = 0; c < G::CPE; ++c) nbad += seen[c] != 1;
+  }
+  if (nbad) printf("T=%zu N=%d ADV=%d SIG=%d: element layout wrong in %ld places\n", sizeof(T), N, (int)ADV, SIG, nbad);
+  return nbad;
+}
+
+// Every tile is walked by exactly one wave, whatever the grid and direction.
+long check_walks()
+{
+  long nbad = 0;
+  const int cases[][2] = {{1, 1}, {7, 3}, {16, 16}, {100, 37}, {4096, 2048}, {16384, 2048}, {16384, 1024},
+                          {5000, 999}, {262144, 2048}, {9, 8}, {2049, 2048}};
+  for (auto& cs : cases)
+    for (int down = 0; down < 2; ++down) {
+      const int ntiles = cs[0], nw = cs[1];
+      int* hits = (int*)calloc(ntiles, sizeof(int));
+      for (int b = 0; b < nw; ++b) {
+        const TileWalk w(b, nw, ntiles, down != 0);
+        int j = w.j0;
+        for (int k = 0; k < w.count; ++k, j = w.next(j)) {
+          const int t = w.tile(j);
+          if (j < 0 || j >= w.len || t < 0 || t >= ntiles) { ++nbad; break; }
+          ++hits[t];
+        }
+      }
+      for (int t = 0; t < ntiles; ++t) nbad += hits[t] != 1;
+      free(hits);
+      if (nbad) { printf("tile walk wrong: ntiles=%d waves=%d down=%d\n", ntiles, nw, down); return nbad; }
+    }
+  return nbad;
+}
+
 template <typename T, int N, bool ADV, int SIG>
 long sweep()
 {
@@ -97,7 +156,10 @@ long sweep()
 int main()
 {
   constexpr int cyc = ksig(1, 2, 0), cst = ksig(-1, -1, -1);
-  long bad = 0;
+  long bad = check_walks();
+  bad += check_layout<double, 4, true, cyc>();
+  bad += check_layout<float, 4, true, cyc>();
+  bad += check_layout<double, 4, false, cst>();
   bad += sweep<double, 4, true, cyc>();
   bad += sweep<double, 4, true, cst>();
   bad += sweep<double, 4, false, cst>();

@@ -200,9 +200,9 @@ STREAM_MESHES = [
 @pytest.mark.parametrize("variant", [56, 57])
 def test_streaming_kernel_vs_oracle(mesh_case, kind, variant):
     """Variants 56 and 57 (one wave per block, persistent over tiles of 16
-    elements of one x-row; 56: two tile buffers, the next tile's copies in
-    flight while a tile computes; 57: one buffer, refilled as the tile's
-    outputs leave, two waves per SIMD)."""
+    elements of one x-row; 56: one tile buffer, refilled as soon as the
+    tile's outputs are in registers, two waves per SIMD; 57: two tile
+    buffers, the next tile's copies in flight while a tile computes)."""
     sizes, periodic = mesh_case
     mesh = G.CartesianMesh(sizes, [1.0 / s for s in sizes], [0.0] * 3, periodic)
     fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))

@@ -3,7 +3,10 @@
 host with hipcc, walks every lane of every copy instruction of every tile of
 a set of meshes (all periodicities and upwind directions, FP64 and FP32,
 advection and mass-only instances) and checks that each source lies in its
-buffer and is 16-byte aligned.  No GPU is used."""
+buffer and is 16-byte aligned; that the element region's swizzled chunk
+layout receives every chunk of every element once, where the kernel's reads
+expect it; and that the kernel's tile walk visits every tile once for a range
+of tile counts and grids, in both directions.  No GPU is used."""
 import os
 import shutil
 import subprocess
