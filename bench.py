@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--no-diffusion", action="store_true", help="skip the 64^3 SIPDG diffusion apply")
     ap.add_argument("--face-ops-only", action="store_true",
                     help="only time the face-centric operators (one JSON object)")
+    ap.add_argument("--product6d-only", action="store_true",
+                    help="only time GenDiL's 6D benchmark operator (5^6 and 10^6 elements; one JSON object)")
+    ap.add_argument("--padapt-classes", action="store_true",
+                    help="only time the 6D p-adaptive apply on the mixed mesh and on one mesh per order (one JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time a plain device copy of the apply's input/output bytes (counter calibration)")
     return ap.parse_args()
@@ -409,6 +413,42 @@ def padapt_6d(steps, warmup, n=8):
     }
 
 
+def padapt_classes(steps, warmup, n=8):
+    """The 6D p-adaptive Kronecker apply (variant 2) on the mixed-order 8^6
+    mesh and on three 8^6 meshes of one order each (p = 1, 2, 3): per-element
+    cost of each order class, and its share of the mixed apply."""
+    import numpy as np
+    import torch
+
+    import gendil_amd as G
+    from gendil_amd import workloads as W
+    from gendil_amd.discretization import MakePAdaptiveSpace
+
+    mesh = G.CartesianMesh([n] * 6, [1.0 / n] * 6, [0.0] * 6, [True, True, True, False, False, False])
+    mixed = W.padaptive_orders(mesh.num_cells, 2024)
+    out = {}
+    for name, orders in [("mixed", mixed)] + [(f"p{p}", np.full(mesh.num_cells, p, dtype=mixed.dtype)) for p in (1, 2, 3)]:
+        fes = MakePAdaptiveSpace(mesh, orders, orders + 1)
+        op = G.MakeAdvectionOperator(fes, None, W.phase_space_6d_affine())
+        op.plan.set_variant(torch.float32, 2)
+        N = op.num_dofs
+        gen = torch.Generator(device="cuda").manual_seed(1)
+        x0 = torch.rand(N, generator=gen, device="cuda", dtype=torch.float32)
+        xs = [x0, x0 * 0.5, x0 * -0.25]
+        ys = [torch.empty_like(x0) for _ in range(3)]
+        ms = time_applies(op, xs, ys, steps, warmup)
+        out[name] = {"ms": ms, "dofs": N, "ns_per_element": ms * 1e6 / mesh.num_cells}
+        del xs, ys, x0, op, fes
+        torch.cuda.empty_cache()
+    counts = {f"p{p}": int((mixed == p).sum()) for p in (1, 2, 3)}
+    est = {k: out[k]["ns_per_element"] * counts[k] * 1e-6 for k in counts}
+    total = sum(est.values())
+    out["mixed_elements_per_order"] = counts
+    out["estimated_share_of_mixed"] = {k: v / total for k, v in est.items()}
+    out["estimated_mixed_ms"] = total
+    return out
+
+
 def product_6d(steps, warmup, n=5):
     """GenDiL's own 6D benchmark operator (benchmarks/benchmark-advection-6d.cpp):
     3D x 2D x 1D product mesh, n elements per axis (h = 1, 1.2345, 1), Legendre
@@ -448,6 +488,31 @@ def product_6d(steps, warmup, n=5):
                                     "kind": "reference"}
     except Exception as e:  # the GPU figure stands on its own
         out["cpu_reference"] = {"error": str(e)[:200]}
+    out["variant"] = op.plan.query_variant(torch.float64, 0.0, 1.0)
+    del xs, ys, x0, op
+    torch.cuda.empty_cache()
+    # the same operator on 10^6 elements (64 M DoFs, 512 MB per vector): the
+    # HBM roofline of the kernel, which the 1 M-DoF benchmark size is too
+    # small to reach
+    big = 10
+    mesh = G.MakeCartesianProductMesh(G.Cartesian3DMesh(1.0, big, big, big), G.Cartesian2DMesh(1.2345, big, big),
+                                      G.Cartesian1DMesh(1.0, big))
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLegendreFiniteElement(1))
+    op = G.MakeAdvectionOperator(fes, G.MakeIntegrationRule(3), VectorField([SeparableField.constant(1.0)] * 6))
+    N = op.num_dofs
+    x0 = torch.rand(N, dtype=torch.float64, device="cuda")
+    xs = [x0, x0 * 0.5, x0 * -0.25]
+    ys = [torch.empty_like(x0) for _ in range(3)]
+    ms = time_applies(op, xs, ys, max(5, steps // 4), 2)
+    out["large"] = {
+        "workload": f"same operator, product mesh {big}^6",
+        "dofs": N,
+        "value": N / (ms * 1e-3) / 1e9,
+        "unit": "GDoF/s",
+        "ms_per_step": ms,
+        "roofline_frac": 16.0 * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+    del xs, ys, x0
     return out
 
 
@@ -593,6 +658,12 @@ def main():
 
     if args.face_ops_only:
         print(json.dumps({"face_ops": face_ops(args.steps, args.warmup)}))
+        return
+    if args.product6d_only:
+        print(json.dumps({"product6d": product_6d(args.steps, args.warmup)}))
+        return
+    if args.padapt_classes:
+        print(json.dumps({"padapt_classes": padapt_classes(args.steps, args.warmup)}))
         return
     n = args.n
     mesh = G.CartesianMesh.unit(3, n, True)

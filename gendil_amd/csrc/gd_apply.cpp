@@ -8,6 +8,7 @@
 #include "gd_dispatch.hpp"
 #include "gd_k3d_api.hpp"
 #include "gd_k3k_api.hpp"
+#include "gd_n2k_api.hpp"
 #include "gd_padapt_api.hpp"
 
 namespace gd {
@@ -69,7 +70,7 @@ template <typename T>
 int k3_choice(const gd_plan* plan, int N, bool adv, bool mass)
 {
   const int tuned = tuned_lanes(plan, N, sizeof(T));
-  if (tuned >= 0) return tuned;
+  if (tuned >= 0 && tuned != n2k::kVariant) return tuned;
   if (N == 4 && k3k::available<T>(plan, 40, adv, mass)) return 16 + 40;
   const int ks = k3k::default_staged(N, sizeof(T));
   if (ks >= 0 && k3k::available<T>(plan, ks, adv, mass)) return 16 + ks;
@@ -85,6 +86,7 @@ int k3_choice(const gd_plan* plan, int N, bool adv, bool mass)
 template <typename T>
 bool k3_eligible(const gd_plan* plan, int choice, bool adv, bool mass)
 {
+  if (choice == n2k::kVariant) return n2k::available<T>(plan, adv, mass);
   if (choice >= 16) return k3k::available<T>(plan, choice - 16, adv, mass);
   return k3::available<T>(plan, choice, adv, mass);
 }
@@ -118,6 +120,13 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
   if constexpr (std::is_same_v<T, float>) {
     if (adv && plan->beta_kind == GD_BETA_POINTWISE)
       return fail(GD_ERR_UNSUPPORTED, "pointwise beta is FP64 only");
+  }
+  // 4D-6D, order 1, constant coefficients: the two-node Kronecker kernel
+  // unless the generic kernel (0) is pinned (measured on MI355X for GenDiL's
+  // 6D benchmark operator, 5^6 elements: see n2k)
+  if (plan->dim >= 4 && tuned_lanes(plan, 2, sizeof(T)) != 0 && n2k::available<T>(plan, adv, mass_coef != T(0))) {
+    const int r = n2k::apply<T>(plan, mass_coef, adv_coef, x, y, s);
+    if (r != GD_ERR_UNSUPPORTED) return r;
   }
   const TableSet* ts = find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
   const Params<T> prm = make_params<T>(plan, ts, mass_coef, adv_coef);
@@ -222,7 +231,7 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
   if (plan->padapt)
     for (int v = 0; v < gd::pa::num_variants(plan->dim); ++v) choices.push_back(v);
   else
-    choices = {0, 1, 2, 4, 17, 18, 20, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57};
+    choices = {0, 1, 2, 4, 17, 18, 20, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, gd::n2k::kVariant};
   const int saved = *knob;
   double best = 1e300;
   int best_lanes = 0;
@@ -282,6 +291,9 @@ int gd_plan_query_variant(const gd_plan* plan, int32_t precision_bytes, double m
   const int N = plan->desc.order + 1;
   auto resolve = [&](auto tag) -> int {
     using T = decltype(tag);
+    if (plan->dim >= 4 && plan->tuned_k3_lanes[sizeof(T) == 8 ? 0 : 1] != 0 &&
+        gd::n2k::available<T>(plan, adv, mass))
+      return gd::n2k::kVariant;
     const int choice = gd::k3_choice<T>(plan, N, adv, mass);
     if (choice >= 16 && gd::k3k::available<T>(plan, choice - 16, adv, mass)) return choice;
     const int lanes = choice >= 16 ? (N == 4 ? (sizeof(T) == 8 ? 2 : 4) : 1) : choice;
@@ -303,7 +315,7 @@ int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes)
     return GD_OK;
   }
   if (lanes != -1 && lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 17 && lanes != 18 &&
-      lanes != 20 && !(lanes >= 48 && lanes <= 57))
+      lanes != 20 && !(lanes >= 48 && lanes <= 57) && lanes != gd::n2k::kVariant)
     return fail(GD_ERR_INVALID, "gd_plan_set_variant: bad arguments");
   plan->tuned_k3_lanes[slot] = lanes;
   return GD_OK;

@@ -57,10 +57,13 @@ struct DCfg {
   // xi = 0, xi = 1) per term and axis, 1 where a term has no factor
   static constexpr int MUS = kMaxTerms * D * (Q + 2);
   static constexpr int PER_ELEM = 2 * ND + SCR + MUS;
-  static constexpr int LDS_BUDGET = 32 * 1024;
+  // measured on MI355X for 3D n = 4, q = 5 FP64 (64^3): 128 threads over
+  // 16 KB of elements 945 us, 256 threads over 32 KB 1000-1050 us, 64
+  // threads 1010-2270 us, 64 KB budgets 1470-2300 us
+  static constexpr int LDS_BUDGET = D == 3 ? 16 * 1024 : 32 * 1024;
   static constexpr int EPB_RAW = LDS_BUDGET / (int)(PER_ELEM * sizeof(T));
   static constexpr int EPB = EPB_RAW < 1 ? 1 : (EPB_RAW > 16 ? 16 : EPB_RAW);
-  static constexpr int NT = D >= 6 ? 128 : 256;
+  static constexpr int NT = D >= 6 || D == 3 ? 128 : 256;
 };
 
 template <class C>

@@ -1,0 +1,157 @@
+// Two-node (p = 1) Kronecker apply for 4D-6D Cartesian meshes with
+// constant coefficients: the mass term sigma M and the upwind advection
+// operator of a constant field beta, any basis and rule of the plan (GenDiL's
+// 6D benchmark operator, benchmark-advection-6d.cpp: Legendre p = 1, three
+// Gauss points, beta = 1).
+//
+// With constant beta and sigma the element operator of the quadrature kernel
+// (advectionoperator.hpp:137-227 and the face terms of :596-676,
+// massoperator.hpp:39-89) is a sum of Kronecker products of 2 x 2 matrices
+// built from the rule's 1D tables (the tensor rule factorises every
+// integral):
+//   Km M x ... x M                       (mass, Km = mass_coef detJ sigma)
+//   M x .. x C_d x .. x M                (cell term of component d and its own
+//                                         outflow face, Kt_d = adv_coef beta_d
+//                                         detJ / h_d: C_d = -Kt_d D + the
+//                                         face's outer product)
+//   M x .. x F_d x .. x M  on the upwind neighbour along d
+// with M = B^T W B, D = Gb^T W B.  All terms share the M factors: factored
+// through M^{-1} along the term's own axis,
+//   y = (M x ... x M) [ Km u + sum_d (M^{-1} C_d)_d u + sum_d (M^{-1} F_d)_d u_nb(d) ],
+// one 2 x 2 contraction per axis and term, then D contractions with M.
+//
+// Layout: one lane per node, 2^D lanes per element (node bits i_0 .. i_{D-1}
+// are the lane bits), 64 / 2^D elements per wave; a contraction along axis d
+// pairs lane l with lane l ^ (1 << d).  Every load and store moves an
+// element's contiguous 2^D values.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+namespace gd {
+namespace n2k {
+
+template <typename T>
+struct N2Params {
+  int E;
+  int M[6];       // elements per axis
+  int stride[6];  // element-index stride of each axis
+  int periodic;   // bit d: axis d periodic
+  int negmask;    // bit d: beta_d < 0 (inflow through the + face)
+  int bc_zero;    // boundary faces carry inflow data g = 0 (else they are skipped)
+  T km;
+  // per axis, row-major 2 x 2: M^{-1} C_d with the own outflow face (ci) and
+  // without it (cb, boundary face without data), M^{-1} F_d (f)
+  T ci[6][4], cb[6][4], f[6][4];
+  T m[4];
+};
+
+// lane ^ MASK of a 64-lane wave (DPP quad permutes for 1 and 2)
+template <int MASK>
+__device__ __forceinline__ float xshfl(float v)
+{
+  if constexpr (MASK == 1)
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xf, 0xf, false));
+  else if constexpr (MASK == 2)
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xf, 0xf, false));
+  else
+    return __shfl_xor(v, MASK, 64);
+}
+template <int MASK>
+__device__ __forceinline__ double xshfl(double v)
+{
+  if constexpr (MASK == 1 || MASK == 2) {
+    constexpr int ctl = MASK == 1 ? 0xB1 : 0x4E;
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, ctl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctl, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+  } else {
+    return __shfl_xor(v, MASK, 64);
+  }
+}
+
+template <int I, int NN, typename F>
+__device__ __forceinline__ void sfor(F&& f)
+{
+  if constexpr (I < NN) {
+    f(std::integral_constant<int, I>{});
+    sfor<I + 1, NN>(f);
+  }
+}
+
+// entries (b, b) and (b, 1 - b) of a row-major 2 x 2 matrix
+template <typename T>
+__device__ __forceinline__ void row_of(const T (&a)[4], int b, T& self, T& other)
+{
+  self = b ? a[3] : a[0];
+  other = b ? a[2] : a[1];
+}
+
+template <typename T, int D>
+__global__ void __launch_bounds__(256) n2k_kernel(N2Params<T> p, const T* __restrict__ x, T* __restrict__ y)
+{
+  constexpr int ND = 1 << D, EPW = 64 / ND;
+  const int lane = (int)threadIdx.x & 63, node = lane & (ND - 1);
+  const int e = ((int)blockIdx.x * ((int)blockDim.x / 64) + (int)threadIdx.x / 64) * EPW + lane / ND;
+  const bool valid = e < p.E;
+  const int ec = valid ? e : p.E - 1;  // lanes past the last element load in bounds, store nothing
+  const T u = x[(long long)ec * ND + node];
+  T w = p.km * u;
+  int rem = ec;
+  sfor<0, D>([&](auto d_) {
+    constexpr int d = decltype(d_)::value;
+    const int Md = p.M[d];
+    const int sd = rem % Md;
+    rem /= Md;
+    const int b = (node >> d) & 1;
+    const bool per = (p.periodic >> d) & 1, neg = (p.negmask >> d) & 1;
+    const bool has_lo = Md == 1 ? per : (sd > 0 || per);
+    const bool has_hi = Md == 1 ? per : (sd < Md - 1 || per);
+    const bool own = neg ? (has_lo || p.bc_zero) : (has_hi || p.bc_zero);
+    const bool inflow = neg ? has_hi : has_lo;
+    // cell term and own outflow face
+    T cs, co;
+    if (own) row_of(p.ci[d], b, cs, co);
+    else row_of(p.cb[d], b, cs, co);
+    const T up = xshfl<(1 << d)>(u);
+    w += cs * u + co * up;
+    // inflow face: the upwind neighbour's pencil along d (wrapping on a
+    // periodic axis); without one, the element's own values with zero weights
+    const int st = p.stride[d];
+    int nb = ec;
+    if (inflow) {
+      if (!neg) nb = sd > 0 ? ec - st : ec + (Md - 1) * st;
+      else nb = sd < Md - 1 ? ec + st : ec - (Md - 1) * st;
+    }
+    const T nv = x[(long long)nb * ND + node];
+    const T np = xshfl<(1 << d)>(nv);
+    T fs, fo;
+    row_of(p.f[d], b, fs, fo);
+    if (!inflow) fs = fo = T(0);
+    w += fs * nv + fo * np;
+  });
+  T ms, mo;
+  sfor<0, D>([&](auto d_) {
+    constexpr int d = decltype(d_)::value;
+    row_of(p.m, (node >> d) & 1, ms, mo);
+    const T wp = xshfl<(1 << d)>(w);
+    w = ms * w + mo * wp;
+  });
+  if (valid) y[(long long)e * ND + node] = w;
+}
+
+template <typename T, int D>
+int launch(const N2Params<T>& prm, const T* x, T* y, hipStream_t s)
+{
+  constexpr int EPB = 4 * (64 >> D);  // four waves per block
+  const int nb = (prm.E + EPB - 1) / EPB;
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL((n2k_kernel<T, D>), dim3((unsigned)nb), dim3(256), 0, s, prm, x, y);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace n2k
+}  // namespace gd

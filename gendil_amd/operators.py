@@ -174,7 +174,8 @@ class Plan:
         (register kernel), 17/18/20 (Kronecker kernel, 16 + lanes), 48..55 (LDS-staged
         Kronecker kernel: +1 outputs staged through LDS, +2 128-thread blocks, +4 one term at
         a time), 56 / 57 (streaming form: one wave per block, persistent; 56 one tile buffer and two
-        waves per SIMD, 57 two tile buffers with the next tile's copies in flight);
+        waves per SIMD, 57 two tile buffers with the next tile's copies in flight), 64 (4D-6D, order 1,
+        constant coefficients: two-node Kronecker kernel);
         p-adaptive plans: the layout index."""
         nbytes = 8 if dtype == torch.float64 else 4
         L.check(L.lib().gd_plan_set_variant(self.handle, nbytes, lanes), "gd_plan_set_variant")

@@ -218,7 +218,9 @@ int gd_adv_hmesh_apply_f64(const gd_plan* coarse, const gd_plan* fine, int32_t f
  * buffer, refilled as soon as a tile's outputs are in registers, two waves
  * per SIMD; 57: two tile buffers, the next tile's copies into LDS in flight
  * while a tile computes, one wave per SIMD; rows must be a multiple of 16
- * elements) where they apply.  p-adaptive plans:
+ * elements) where they apply; 4D-6D plans of order 1 with constant
+ * coefficients: 64 = the two-node Kronecker kernel (one lane per node).
+ * p-adaptive plans:
  * the thread-group layouts of the quadrature kernel (0 = one wave per n=3
  * element, 1 = one block per n=3 element; n=4 elements take a block, n=2
  * elements a wave) and, in 6D for a field with paired affine components of
@@ -229,7 +231,8 @@ int gd_plan_autotune(gd_plan* plan, int32_t precision_bytes, double mass_coef, d
                      int32_t reps, int32_t* chosen, double* best_ms);
 /* Pin a variant: -1 (default); uniform plans 0 (generic), 1, 2 or 4 lanes
  * (register kernel), 17, 18 or 20 (Kronecker kernel), 48..55 (LDS-staged
- * Kronecker kernel), 56 or 57 (streaming Kronecker kernel); p-adaptive plans the variant index as reported by
+ * Kronecker kernel), 56 or 57 (streaming Kronecker kernel), 64 (two-node
+ * Kronecker kernel, 4D-6D); p-adaptive plans the variant index as reported by
  * gd_plan_autotune.  A pinned variant that does not serve a later apply
  * falls back to the default for that apply. */
 int gd_plan_set_variant(gd_plan* plan, int32_t precision_bytes, int32_t lanes);
