@@ -84,7 +84,7 @@ bool params(const gd_plan* plan, double mass_coef, double adv_coef, N2Params<T>&
   p.periodic = 0;
   p.negmask = 0;
   p.bc_zero = plan->bc_mode == GD_BC_INFLOW ? 1 : 0;
-  p.km = (T)(mass_coef * detJ * g.sigma);
+  p.tab[76] = (T)(mass_coef * detJ * g.sigma);
   int st = 1;
   for (int d = 0; d < 6; ++d) {
     p.M[d] = d < D ? (int)plan->desc.sizes[d] : 1;
@@ -92,7 +92,14 @@ bool params(const gd_plan* plan, double mass_coef, double adv_coef, N2Params<T>&
     st *= p.M[d];
     if (d < D && plan->desc.periodic[d]) p.periodic |= 1 << d;
   }
-  for (int i = 0; i < 4; ++i) p.m[i] = (T)Mm[i];
+  // row (b, b), (b, 1 - b) of a row-major 2 x 2 matrix at dst + 2 b
+  auto rows = [&](const M2 a, T* dst) {
+    dst[0] = (T)a[0];
+    dst[1] = (T)a[1];
+    dst[2] = (T)a[3];
+    dst[3] = (T)a[2];
+  };
+  rows(Mm, p.tab + 72);
   for (int d = 0; d < D; ++d) {
     const double kt = adv ? adv_coef * g.beta[d] * detJ / plan->desc.h[d] : 0.0;
     const bool neg = g.beta[d] < 0.0;
@@ -110,11 +117,9 @@ bool params(const gd_plan* plan, double mass_coef, double adv_coef, N2Params<T>&
     mul(Mi, cb, cb);
     mul(Mi, ci, ci);
     mul(Mi, f, f);
-    for (int i = 0; i < 4; ++i) {
-      p.cb[d][i] = (T)cb[i];
-      p.ci[d][i] = (T)ci[i];
-      p.f[d][i] = (T)f[i];
-    }
+    rows(cb, p.tab + d * 12);
+    rows(ci, p.tab + d * 12 + 4);
+    rows(f, p.tab + d * 12 + 8);
   }
   return true;
 }

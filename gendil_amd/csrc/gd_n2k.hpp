@@ -41,11 +41,13 @@ struct N2Params {
   int periodic;   // bit d: axis d periodic
   int negmask;    // bit d: beta_d < 0 (inflow through the + face)
   int bc_zero;    // boundary faces carry inflow data g = 0 (else they are skipped)
-  T km;
-  // per axis, row-major 2 x 2: M^{-1} C_d with the own outflow face (ci) and
-  // without it (cb, boundary face without data), M^{-1} F_d (f)
-  T ci[6][4], cb[6][4], f[6][4];
-  T m[4];
+  // The lane coefficients, laid out for the lane that reads them: per axis d
+  // (12 values from d * 12) the row (b, b), (b, 1 - b) of M^{-1} C_d
+  // without (own = 0) and with (own = 1) the own outflow face at
+  // 4 own + 2 b, then the row of M^{-1} F_d at 8 + 2 b; from 72 the rows of
+  // M at 72 + 2 b, and Km at 76.
+  static constexpr int kTab = 77;
+  T tab[kTab];
 };
 
 // lane ^ MASK of a 64-lane wave (DPP quad permutes for 1 and 2)
@@ -82,24 +84,25 @@ __device__ __forceinline__ void sfor(F&& f)
   }
 }
 
-// entries (b, b) and (b, 1 - b) of a row-major 2 x 2 matrix
-template <typename T>
-__device__ __forceinline__ void row_of(const T (&a)[4], int b, T& self, T& other)
-{
-  self = b ? a[3] : a[0];
-  other = b ? a[2] : a[1];
-}
-
 template <typename T, int D>
 __global__ void __launch_bounds__(256) n2k_kernel(N2Params<T> p, const T* __restrict__ x, T* __restrict__ y)
 {
   constexpr int ND = 1 << D, EPW = 64 / ND;
+  // the coefficient table in LDS: each lane reads its own rows there (held in
+  // scalar registers the 77 values would not fit)
+  __shared__ T tab[N2Params<T>::kTab];
+  for (int i = (int)threadIdx.x; i < N2Params<T>::kTab; i += (int)blockDim.x) tab[i] = p.tab[i];
+  __syncthreads();
   const int lane = (int)threadIdx.x & 63, node = lane & (ND - 1);
-  const int e = ((int)blockIdx.x * ((int)blockDim.x / 64) + (int)threadIdx.x / 64) * EPW + lane / ND;
+  // (blocks in launch order: one contiguous run of elements per XCD halves
+  // the fetched bytes on a 10^6 mesh but measured 15 % slower there, and
+  // equal at 5^6)
+  int e = ((int)blockIdx.x * ((int)blockDim.x / 64) + (int)threadIdx.x / 64) * EPW + lane / ND;
+  if constexpr (EPW == 1) e = __builtin_amdgcn_readfirstlane(e);  // one element per wave: scalar index work
   const bool valid = e < p.E;
   const int ec = valid ? e : p.E - 1;  // lanes past the last element load in bounds, store nothing
   const T u = x[(long long)ec * ND + node];
-  T w = p.km * u;
+  T w = tab[76] * u;
   int rem = ec;
   sfor<0, D>([&](auto d_) {
     constexpr int d = decltype(d_)::value;
@@ -113,9 +116,8 @@ __global__ void __launch_bounds__(256) n2k_kernel(N2Params<T> p, const T* __rest
     const bool own = neg ? (has_lo || p.bc_zero) : (has_hi || p.bc_zero);
     const bool inflow = neg ? has_hi : has_lo;
     // cell term and own outflow face
-    T cs, co;
-    if (own) row_of(p.ci[d], b, cs, co);
-    else row_of(p.cb[d], b, cs, co);
+    const T* cr = tab + d * 12 + (own ? 4 : 0) + 2 * b;
+    const T cs = cr[0], co = cr[1];
     const T up = xshfl<(1 << d)>(u);
     w += cs * u + co * up;
     // inflow face: the upwind neighbour's pencil along d (wrapping on a
@@ -128,15 +130,14 @@ __global__ void __launch_bounds__(256) n2k_kernel(N2Params<T> p, const T* __rest
     }
     const T nv = x[(long long)nb * ND + node];
     const T np = xshfl<(1 << d)>(nv);
-    T fs, fo;
-    row_of(p.f[d], b, fs, fo);
-    if (!inflow) fs = fo = T(0);
+    const T* fr = tab + d * 12 + 8 + 2 * b;
+    const T fs = inflow ? fr[0] : T(0), fo = inflow ? fr[1] : T(0);
     w += fs * nv + fo * np;
   });
-  T ms, mo;
   sfor<0, D>([&](auto d_) {
     constexpr int d = decltype(d_)::value;
-    row_of(p.m, (node >> d) & 1, ms, mo);
+    const T* mr = tab + 72 + 2 * ((node >> d) & 1);
+    const T ms = mr[0], mo = mr[1];
     const T wp = xshfl<(1 << d)>(w);
     w = ms * w + mo * wp;
   });
