@@ -134,6 +134,10 @@ struct StreamGeom {
   static constexpr int INS = U_INS + S_INS;     // copies per tile and lane
   static constexpr int ST = TW * CPE / 64;      // output stores per tile and lane
   static constexpr int BUF_CHUNKS = INS * 64;
+  // one-buffer form: the outputs are staged in the second region, which
+  // therefore spans at least TW * UCS chunks
+  static constexpr int SREG = S_INS * 64 > TW * UCS ? S_INS * 64 : (TW * UCS + 63) / 64 * 64;
+  static constexpr int BUF1_CHUNKS = U_INS * 64 + SREG;
   static_assert(N3 % VW == 0 && N2 % VW == 0, "element planes must be whole 16-byte chunks");
   static_assert((TW * CPE) % 64 == 0, "a tile's outputs are whole wave-wide stores");
   static_assert(INS + ST < 64, "vmcnt counts to 63");
@@ -203,12 +207,12 @@ constexpr unsigned stream_kinds(int j)
 // bases b0..b7 by the lane's fixed kind (indices fixed at compile time: a
 // run-time index into a local array would place it in scratch memory, and a
 // scratch read would wait for every copy in flight).
-template <typename G, typename T, int J>
+template <typename G, typename T, int J, int END = G::INS>
 __device__ __forceinline__ void stream_issue(const int (&kind)[G::INS], const int (&off)[G::INS], unsigned base,
                                              const T* b0, const T* b1, const T* b2, const T* b3, const T* b4,
                                              const T* b5, const T* b6, const T* b7)
 {
-  if constexpr (J < G::INS) {
+  if constexpr (J < END) {
     constexpr unsigned ks = stream_kinds<G>(J);
     // the lane's kind is re-read every tile (the empty asm), so its
     // comparisons are not kept as loop-invariant masks in scalar registers
@@ -223,7 +227,7 @@ __device__ __forceinline__ void stream_issue(const int (&kind)[G::INS], const in
     if constexpr ((ks >> 6) & 1) src = kd == 6 ? b6 : src;
     if constexpr ((ks >> 7) & 1) src = kd == 7 ? b7 : src;
     dma16(src + off[J], base + J * 1024);
-    stream_issue<G, T, J + 1>(kind, off, base, b0, b1, b2, b3, b4, b5, b6, b7);
+    stream_issue<G, T, J + 1, END>(kind, off, base, b0, b1, b2, b3, b4, b5, b6, b7);
   }
 }
 
@@ -270,7 +274,7 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
   using KT = KTab<N>;
   using C = Chunk<T, VW>;
   static_assert(NB == 1 || NB == 2, "one or two tile buffers");
-  __shared__ C lds[NB * G::BUF_CHUNKS];
+  __shared__ C lds[NB == 1 ? G::BUF1_CHUNKS : 2 * G::BUF_CHUNKS];
   const unsigned lds_base = (unsigned)(size_t)(__attribute__((address_space(3))) void*)lds;
 
   const int lane = (int)threadIdx.x;
@@ -325,7 +329,9 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
   };
 
   // ---- the copies of one tile into buffer b ----
-  auto issue = [&](int t, int b) __attribute__((always_inline)) {
+  // part 0: every copy of the tile; 1: the element region (elements and
+  // halo, U_INS copies); 2: the second region (slabs and tables)
+  auto issue = [&](int t, int b, int part) __attribute__((always_inline)) {
     const unsigned base = lds_base + (unsigned)(b * G::BUF_CHUNKS * (int)sizeof(C));
     const TilePos tp = tile_pos(t);
     // per-tile bases (uniform), selected per lane by its fixed kind
@@ -352,7 +358,12 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
       bt[sl] = p.tab[G::SC[sl]] + (long long)pos * N2;
     }
     const T* b7 = G::NTZ > 0 ? p.tab[G::ZC] + (long long)tp.sz * N2 : b0;
-    stream_issue<G, T, 0>(kind, off, base, b0, b1, b2, b3, bt[0], bt[1], bt[2], b7);
+    if (part == 1)
+      stream_issue<G, T, 0, G::U_INS>(kind, off, base, b0, b1, b2, b3, bt[0], bt[1], bt[2], b7);
+    else if (part == 2)
+      stream_issue<G, T, G::U_INS, G::INS>(kind, off, base, b0, b1, b2, b3, bt[0], bt[1], bt[2], b7);
+    else
+      stream_issue<G, T, 0>(kind, off, base, b0, b1, b2, b3, bt[0], bt[1], bt[2], b7);
   };
 
   // ---- one tile: compute from buffer b while the next tile moves into b^1 ----
@@ -392,7 +403,7 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
     // this tile's copies only: the stores and the next copies stay in flight.
     if constexpr (NB == 2) {
       if (tn >= 0) {
-        issue(tn, b ^ 1);
+        issue(tn, b ^ 1, 0);
         if (first) wait_vmcnt<G::INS>();
         else wait_vmcnt<G::INS + G::ST>();
       } else {
@@ -400,8 +411,9 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
         else wait_vmcnt<G::ST>();
       }
     } else {
-      // one buffer: in flight, oldest first, this tile's copies and the
-      // previous tile's stores (issued after them)
+      // one buffer: in flight, oldest first, this tile's element copies
+      // (issued halfway through the previous tile), its slab and table
+      // copies, and the previous tile's stores (issued after them)
       if (first) wait_vmcnt<0>();
       else wait_vmcnt<G::ST>();
     }
@@ -491,6 +503,15 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
           }
         }
       }
+      if constexpr (NB == 1) {
+        // the element region (planes and x halo) has been read for this
+        // tile: once every read has landed (lgkmcnt(0)), refill it with the
+        // next tile's elements while the rest of the tile computes from
+        // registers and the second region
+        __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));
+        __builtin_amdgcn_wave_barrier();
+        if (tn >= 0) issue(tn, 0, 1);
+      }
       if constexpr (ADV) {
         // x-neighbour slab through this lane's z-factor row: the element's
         // lanes exchange their rows
@@ -542,13 +563,17 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
       }
     }
 
-    // outputs through the tile's element region: every lane has finished
-    // reading `cur` (the values were consumed), so the wave may overwrite it
+    // outputs staged through LDS: with two buffers through the tile's
+    // element region, with one through the second region (the element
+    // region already holds the next tile's copies); every lane has finished
+    // reading it (the values were consumed), so the wave may overwrite it
+    C* const stg = NB == 1 ? cur + G::U_INS * 64 : cur;
+    static_assert(NB == 2 || TW * UCS <= G::SREG, "the outputs fit the second region");
     __builtin_amdgcn_wave_barrier();
     {
       // PX is even: chunks 2k and 2k+1 stay neighbours, at (2k) ^ px
       static_assert(G::PC % 2 == 0, "pairs of chunks per plane");
-      C* ow = cur + el * UCS + h * G::PC;
+      C* ow = stg + el * UCS + h * G::PC;
       const int px = G::PX(h);
 #pragma unroll
       for (int k = 0; k < G::PC / 2; ++k) {
@@ -575,25 +600,25 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
       }
     } else {
       // one buffer: take the staged outputs into registers, let every LDS
-      // read finish (lgkmcnt(0)), then refill the buffer with the next
-      // tile's copies before the stores go out
+      // read finish (lgkmcnt(0)), then refill the second region with the
+      // next tile's slabs and tables before the stores go out
       C st[G::ST];
 #pragma unroll
       for (int i = 0; i < G::ST; ++i) {
         const int cidx = i * 64 + lane;
         const int ee = cidx / CPE;
-        st[i] = cur[ee * UCS + G::swz(cidx - ee * CPE)];
+        st[i] = stg[ee * UCS + G::swz(cidx - ee * CPE)];
       }
       __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));
       __builtin_amdgcn_wave_barrier();
-      if (tn >= 0) issue(tn, 0);
+      if (tn >= 0) issue(tn, 0, 2);
 #pragma unroll
       for (int i = 0; i < G::ST; ++i) store_nt(&dst[i * 64 + lane], st[i]);
     }
     __builtin_amdgcn_wave_barrier();
   };
 
-  if (walk.count > 0) issue(walk.tile(j), 0);
+  if (walk.count > 0) issue(walk.tile(j), 0, 0);
   for (int k = 0, b = 0; k < walk.count; ++k, b = NB == 2 ? b ^ 1 : 0) {
     const int jn = walk.next(j);
     step(walk.tile(j), k + 1 < walk.count ? walk.tile(jn) : -1, b, k == 0);
