@@ -13,17 +13,19 @@
 //   chunk, so the 16 elements a wave reads at one offset spread over the
 //   banks); lanes that fall on a pad chunk or past the tile copy an in-bounds
 //   chunk that is never read.
-// * NB = 1 (two waves per SIMD): one buffer, refilled with the next tile's
-//   copies as soon as the staged outputs are in registers; the stores follow
-//   the copies.  NB = 2 (one wave per SIMD): while tile t computes from one
-//   buffer, tile t+1's copies fill the other.
+// * NB = 1 (two waves per SIMD): one buffer, refilled in two parts: its
+//   element region as soon as the tile's element planes have been read (the
+//   first phase), its second region (slabs and tables) once the outputs,
+//   staged there, are in registers; the stores follow the copies.  NB = 2
+//   (one wave per SIMD): while tile t computes from one buffer, tile t+1's
+//   copies fill the other.
 // * Wave-local: no block barrier at all.  The wave waits for a tile's copies
 //   with s_waitcnt vmcnt (vector memory operations complete in issue order),
 //   leaving the previous tile's output stores (and, with two buffers, the
 //   next tile's copies) in flight; no other vector memory operation is
 //   issued in the loop.
-// * Outputs are staged through the tile's element region and leave as
-//   coalesced 16-byte non-temporal stores.
+// * Outputs are staged through LDS (NB = 2: the tile's element region; NB = 1:
+//   the second region) and leave as coalesced 16-byte non-temporal stores.
 // Each XCD walks one contiguous chunk of tiles (TileWalk), so the y and z
 // neighbours of a tile were read by the same XCD shortly before and are
 // still in its L2.
