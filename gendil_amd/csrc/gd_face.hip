@@ -22,6 +22,7 @@
 
 #include "gendil_amd.h"
 #include "gd_internal.hpp"
+#include "gd_k3k_api.hpp"
 
 #include <algorithm>
 #include <string>
@@ -571,6 +572,13 @@ extern "C" int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, doubl
     return gd::fail(GD_ERR_INVALID, "gd_adv_face_apply_f64: x and y must not overlap");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int D = plan->dim;
+  // 3D GLL, q = n, separable one-axis beta, rows of whole tiles: the
+  // streaming Kronecker kernel with its cell terms off, accumulating
+  // (variant 0 pinned for FP64 keeps the face kernels below)
+  if (D == 3 && plan->tuned_k3_lanes[0] != 0) {
+    const int r = gd::k3k::apply_faces<double>(plan, x, y, st);
+    if (r != GD_ERR_UNSUPPORTED) return r;
+  }
   {
     Args a;
     if (int rc = common_args(plan, plan, 0, -1, a)) return rc;

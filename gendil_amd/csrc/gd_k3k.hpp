@@ -48,6 +48,7 @@ struct KParams {
   int negmask;     // bit d: beta_d <= 0 at every face point (else >= 0)
   int num_blocks;
   T Kt[3];         // adv_coef detJ / h_d, times beta_d's value when beta_d is constant
+  T Kc[3];         // the same for the cell terms (0: the face terms alone)
   T Km;            // mass_coef detJ sigma
   const T* tab[3];    // [M_axis][N][N]: B^T W[beta_d] B, or Gb^T W[beta_d] B when beta_d varies along d
   const T* fsamp[3];  // [M_d][2]: beta_d at xi = 0 and xi = 1, when beta_d varies along d
@@ -291,7 +292,7 @@ __device__ __forceinline__ void inplane(const KParams<T>& p, const int sub[3], c
 #pragma unroll
       for (int c = 0; c < N; ++c) {
         F1[r][c] = AX0 == 1 ? tabm(0, sub[1], r, c) : (T)KT::M.a[r][c];
-        F0[r][c] = -p.Kt[0] * (AX0 == 0 ? tabm(0, sub[0], r, c) : (T)KT::D.a[r][c]);
+        F0[r][c] = -p.Kc[0] * (AX0 == 0 ? tabm(0, sub[0], r, c) : (T)KT::D.a[r][c]);
       }
     T t1[N2], s1[N];
 #pragma unroll
@@ -356,7 +357,7 @@ __device__ __forceinline__ void inplane(const KParams<T>& p, const int sub[3], c
 #pragma unroll
       for (int c = 0; c < N; ++c) {
         F0[r][c] = AX1 == 0 ? tabm(1, sub[0], r, c) : (T)KT::M.a[r][c];
-        F1[r][c] = -p.Kt[1] * (AX1 == 1 ? tabm(1, sub[1], r, c) : (T)KT::D.a[r][c]);
+        F1[r][c] = -p.Kc[1] * (AX1 == 1 ? tabm(1, sub[1], r, c) : (T)KT::D.a[r][c]);
       }
     T t1[N2], s1[N];
 #pragma unroll
@@ -465,7 +466,7 @@ __global__ void __launch_bounds__(NT) k3k_apply(KParams<T> p, const T* __restric
         T dz;
         if constexpr (AX2 == 2) dz = p.tab[2][(sub[2] * N + row) * N + j];
         else dz = lrow<N, LANES, PL, T>(KT::D, h, kk, j);
-        zZ[kk][j] = -p.Kt[2] * dz + ((row == N - 1 && j == N - 1) ? cPown[2] : T(0)) +
+        zZ[kk][j] = -p.Kc[2] * dz + ((row == N - 1 && j == N - 1) ? cPown[2] : T(0)) +
                     ((row == 0 && j == 0) ? cMown[2] : T(0));
       }
     }
@@ -601,7 +602,7 @@ __device__ __forceinline__ void term_x(const KParams<T>& p, const FaceCoefs<T>& 
       T a = T(0);
 #pragma unroll
       for (int ii = 0; ii < N; ++ii) a += kmat<T, N>(AX0 == 0, ft, KT::D, i, ii) * t1[ii];
-      out[j * N + i] -= p.Kt[0] * a;
+      out[j * N + i] -= p.Kc[0] * a;
     }
     out[j * N + N - 1] += fc.cPown(0) * t1[N - 1] + fc.cPnb(0) * s1;
     out[j * N] += fc.cMown(0) * t1[0] + fc.cMnb(0) * s1;
@@ -632,7 +633,7 @@ __device__ __forceinline__ void term_y(const KParams<T>& p, const FaceCoefs<T>& 
       T a = T(0);
 #pragma unroll
       for (int jj = 0; jj < N; ++jj) a += kmat<T, N>(AX1 == 1, ft, KT::D, j, jj) * t1[jj];
-      out[j * N + i] -= p.Kt[1] * a;
+      out[j * N + i] -= p.Kc[1] * a;
     }
     out[(N - 1) * N + i] += fc.cPown(1) * t1[N - 1] + fc.cPnb(1) * s1;
     out[i] += fc.cMown(1) * t1[0] + fc.cMnb(1) * s1;
@@ -830,7 +831,7 @@ __global__ void __launch_bounds__(EPB * N, (SEQ || sizeof(T) == 4) ? 3 : 2)
     T dz;
     if constexpr (TZ[2]) dz = zt[j];
     else dz = lrow<N, N, 1, T>(KT::D, h, 0, j);
-    return -p.Kt[2] * dz + ((h == N - 1 && j == N - 1) ? fc.cPown(2) : T(0)) +
+    return -p.Kc[2] * dz + ((h == N - 1 && j == N - 1) ? fc.cPown(2) : T(0)) +
            ((h == 0 && j == 0) ? fc.cMown(2) : T(0));
   };
   auto coefM = [&](int j) -> T { return lrow<N, N, 1, T>(KT::M, h, 0, j); };

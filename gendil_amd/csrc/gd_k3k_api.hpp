@@ -51,6 +51,13 @@ int apply(const gd_plan* plan, int lanes, T mass_coef, T adv_coef, const T* x, T
 template <typename T>
 bool available(const gd_plan* plan, int lanes, bool adv, bool mass);
 
+// y += the interior-face terms of the advection operator (GenDiL's
+// AdvectionFaceOperator over the interior faces; no cell terms, no faces
+// across a periodic wrap or on the boundary) by the streaming kernel;
+// GD_ERR_UNSUPPORTED (nothing launched) where it does not serve the plan.
+template <typename T>
+int apply_faces(const gd_plan* plan, const T* x, T* y, hipStream_t s);
+
 // Lane count of the register-resident Kronecker variant for n nodes per
 // dimension, and the code (32..39) of the default LDS-staged variant (-1: none
 // compiled for n).

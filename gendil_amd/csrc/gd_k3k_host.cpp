@@ -171,7 +171,8 @@ bool prepare(const gd_plan* plan, int lanes, bool adv, bool mass, KParams<T>& pr
   if (plan->dim != 3 || plan->padapt || plan->desc.basis != GD_BASIS_GLL || plan->desc.num_quad != N) return false;
   if (plan->E >= (int64_t)1 << 31) return false;
   // the streaming kernel deals tiles of 64 / N elements of one x-row
-  if ((lanes == 40 || lanes == 41) && (64 % N != 0 || plan->desc.sizes[0] % (64 / N) != 0)) return false;
+  if ((lanes == 40 || lanes == 41 || lanes == 42) && (64 % N != 0 || plan->desc.sizes[0] % (64 / N) != 0))
+    return false;
   double sigma = 0.0;
   if (mass && !constant_of(plan->sigma_host, &sigma)) return false;
   int sigc = ksig(-1, -1, -1);
@@ -210,6 +211,7 @@ bool prepare(const gd_plan* plan, int lanes, bool adv, bool mass, KParams<T>& pr
       double K = detJ / plan->desc.h[d];
       if (st && st->axis[d] < 0) K *= st->cval[d];
       prm.Kt[d] = (T)K;
+      prm.Kc[d] = (T)K;
       prm.tab[d] = st ? static_cast<const T*>(st->tab[slot][d]) : nullptr;
       prm.fsamp[d] = st ? static_cast<const T*>(st->fs[slot][d]) : nullptr;
     }
@@ -239,12 +241,31 @@ int apply(const gd_plan* plan, int lanes, T mass_coef, T adv_coef, const T* x, T
   // the LDS-staged variants move 16-byte chunks
   if (lanes >= 32 && ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15) != 0)
     return GD_ERR_UNSUPPORTED;
-  for (int d = 0; d < 3; ++d) prm.Kt[d] *= adv_coef;
+  for (int d = 0; d < 3; ++d) {
+    prm.Kt[d] *= adv_coef;
+    prm.Kc[d] *= adv_coef;
+  }
   prm.Km *= mass_coef;
   return fn(prm, x, y, s);
 }
 
+template <typename T>
+int apply_faces(const gd_plan* plan, const T* x, T* y, hipStream_t s)
+{
+  KParams<T> prm{};
+  KLaunch<T> fn = nullptr;
+  if (!prepare<T>(plan, 42, true, false, prm, &fn)) return GD_ERR_UNSUPPORTED;
+  if (((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15) != 0) return GD_ERR_UNSUPPORTED;
+  // the interior faces alone: no cell terms, no faces across a periodic
+  // wrap, no boundary faces
+  prm.periodic = 0;
+  prm.bc_zero = 0;
+  for (int d = 0; d < 3; ++d) prm.Kc[d] = T(0);
+  return fn(prm, x, y, s);
+}
+
 template bool available<double>(const gd_plan*, int, bool, bool);
+template int apply_faces<double>(const gd_plan*, const double*, double*, hipStream_t);
 template bool available<float>(const gd_plan*, int, bool, bool);
 template int apply<double>(const gd_plan*, int, double, double, const double*, double*, hipStream_t);
 template int apply<float>(const gd_plan*, int, float, float, const float*, float*, hipStream_t);

@@ -41,10 +41,10 @@ int launch_staged(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
 }
 
 // Streaming kernel (lanes code 39 + NB: 40 one tile buffer per wave, two
-// waves per SIMD; 41 two buffers, one wave per SIMD):
-// one wave per block, persistent over tiles of one x-row, as many waves as
-// fit on the device at once.
-template <typename T, int N, bool ADV, bool MASS, int SIG, int NB>
+// waves per SIMD; 41 two buffers, one wave per SIMD; 42 one buffer, y +=
+// the operator): one wave per block, persistent over tiles of one x-row, as
+// many waves as fit on the device at once.
+template <typename T, int N, bool ADV, bool MASS, int SIG, int NB, bool ACC = false>
 int launch_stream(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
 {
   using G = StreamGeom<T, N, ADV, SIG>;
@@ -57,13 +57,13 @@ int launch_stream(const KParams<T>& prm_in, const T* x, T* y, hipStream_t s)
     GD_HIP(hipGetDevice(&dev));
     GD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     GD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k3k_stream<T, N, ADV, MASS, SIG, NB>), 64, 0));
+        &per_cu, reinterpret_cast<const void*>(k3k_stream<T, N, ADV, MASS, SIG, NB, ACC>), 64, 0));
     waves = cus * (per_cu > 0 ? per_cu : 1);
   }
   KParams<T> prm = prm_in;
   const int grid = ntiles < waves ? ntiles : waves;
   prm.num_blocks = grid;
-  hipLaunchKernelGGL((k3k_stream<T, N, ADV, MASS, SIG, NB>), dim3((unsigned)grid), dim3(64), 0, s, prm, x, y,
+  hipLaunchKernelGGL((k3k_stream<T, N, ADV, MASS, SIG, NB, ACC>), dim3((unsigned)grid), dim3(64), 0, s, prm, x, y,
                      ntiles);
   GD_HIP(hipGetLastError());
   return GD_OK;
@@ -93,6 +93,10 @@ constexpr int kSigConst = ksig(-1, -1, -1);
   GD_K3K_STREAM_ENTRY(T, N, NB, 1, 0, kSigCyclic), GD_K3K_STREAM_ENTRY(T, N, NB, 1, 1, kSigCyclic),             \
       GD_K3K_STREAM_ENTRY(T, N, NB, 1, 0, kSigConst), GD_K3K_STREAM_ENTRY(T, N, NB, 1, 1, kSigConst),           \
       GD_K3K_STREAM_ENTRY(T, N, NB, 0, 1, kSigConst)
+// y += the operator, one buffer (the face terms alone: the face operator)
+#define GD_K3K_STREAM_ACC(T, N)                                                                  \
+  KEntry<T>{N, 42, 1, 0, kSigCyclic, &launch_stream<T, N, true, false, kSigCyclic, 1, true>},    \
+      KEntry<T> { N, 42, 1, 0, kSigConst, &launch_stream<T, N, true, false, kSigConst, 1, true> }
 #define GD_K3K_STAGED_ALL(T, N)                                                                        \
   GD_K3K_STAGED(T, N, 0), GD_K3K_STAGED(T, N, 1), GD_K3K_STAGED(T, N, 2), GD_K3K_STAGED(T, N, 3),          \
       GD_K3K_STAGED(T, N, 4), GD_K3K_STAGED(T, N, 5), GD_K3K_STAGED(T, N, 6), GD_K3K_STAGED(T, N, 7)

@@ -263,8 +263,10 @@ struct TileWalk {
 
 // Elements are dealt in tiles of TW consecutive elements of one x-row; the
 // host requires sizes[0] to be a multiple of TW (so every tile is full) and
-// no component varying along its own axis.
-template <typename T, int N, bool ADV, bool MASS, int SIG, int NB>
+// no component varying along its own axis.  ACC (one buffer only): y += the
+// operator (the tile's old outputs come into a third LDS region with the
+// tile's other copies) instead of y = the operator.
+template <typename T, int N, bool ADV, bool MASS, int SIG, int NB, bool ACC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 2 : 1)))
 k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
 {
@@ -276,7 +278,11 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
   using KT = KTab<N>;
   using C = Chunk<T, VW>;
   static_assert(NB == 1 || NB == 2, "one or two tile buffers");
-  __shared__ C lds[NB == 1 ? G::BUF1_CHUNKS : 2 * G::BUF_CHUNKS];
+  static_assert(!ACC || NB == 1, "accumulation runs with one buffer");
+  static_assert(G::INS + 2 * G::ST < 64, "vmcnt counts to 63");
+  // one buffer: element region, second region, then (ACC) the old outputs
+  constexpr int YREG0 = G::BUF1_CHUNKS;
+  __shared__ C lds[NB == 1 ? G::BUF1_CHUNKS + (ACC ? G::ST * 64 : 0) : 2 * G::BUF_CHUNKS];
   const unsigned lds_base = (unsigned)(size_t)(__attribute__((address_space(3))) void*)lds;
 
   const int lane = (int)threadIdx.x;
@@ -366,6 +372,15 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
       stream_issue<G, T, G::U_INS, G::INS>(kind, off, base, b0, b1, b2, b3, bt[0], bt[1], bt[2], b7);
     else
       stream_issue<G, T, 0>(kind, off, base, b0, b1, b2, b3, bt[0], bt[1], bt[2], b7);
+    if constexpr (ACC) {
+      // the tile's old outputs, chunk i * 64 + lane to the same LDS chunk
+      if (part != 1) {
+        const T* yt = y + (long long)tp.e0 * N3;
+#pragma unroll
+        for (int i = 0; i < G::ST; ++i)
+          dma16(yt + (i * 64 + lane) * VW, lds_base + (unsigned)((YREG0 + i * 64) * (int)sizeof(C)));
+      }
+    }
   };
 
   // ---- one tile: compute from buffer b while the next tile moves into b^1 ----
@@ -460,7 +475,7 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
       T dz;
       if constexpr (G::TZ[2]) dz = zt[j];
       else dz = rowD[j];
-      return -p.Kt[2] * dz + ((h == N - 1 && j == N - 1) ? fc.cPown(2) : T(0)) +
+      return -p.Kc[2] * dz + ((h == N - 1 && j == N - 1) ? fc.cPown(2) : T(0)) +
              ((h == 0 && j == 0) ? fc.cMown(2) : T(0));
     };
     auto coefM = [&](int j) -> T { return rowM[j]; };
@@ -610,6 +625,12 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
         const int cidx = i * 64 + lane;
         const int ee = cidx / CPE;
         st[i] = stg[ee * UCS + G::swz(cidx - ee * CPE)];
+        if constexpr (ACC) {
+          // plus the old outputs, read before the next tile's copies land
+          const C yo = lds[YREG0 + cidx];
+#pragma unroll
+          for (int l = 0; l < VW; ++l) st[i].v[l] += yo.v[l];
+        }
       }
       __builtin_amdgcn_s_waitcnt((15) | (3 << 14) | (7 << 4) | (0 << 8));
       __builtin_amdgcn_wave_barrier();

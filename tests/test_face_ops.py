@@ -301,3 +301,37 @@ def test_face_gpu_degenerate_meshes(sizes):
         assert np.abs(got).max() == 0.0
     else:
         assert rel_l2(got, ref) <= F64_TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize(
+    "sizes,periodic,coef",
+    [((16, 4, 3), False, "c2"), ((32, 2, 2), True, "c2"), ((16, 3, 2), False, "const"), ((48, 1, 3), False, "c2")],
+)
+def test_face_gpu_streaming_path_matches_face_kernel_and_oracle(sizes, periodic, coef):
+    # 3D GLL p=3, q=4, rows of whole 16-element tiles: the interior faces run
+    # on the streaming Kronecker kernel (cell terms off, y accumulated); the
+    # same operator with variant 0 pinned runs the face kernel
+    import torch
+    from cases import beta_field
+
+    print(f"face streaming path {sizes} periodic={periodic} {coef}", flush=True)
+    h = [1.0 / s for s in sizes]
+    mesh = G.CartesianMesh(list(sizes), h, [0.0] * 3, [periodic] * 3)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    op = G.MakeAdvectionFaceOperator(fes, G.MakeCartesianInteriorFaceConnectivity(fes.mesh.sizes),
+                                     G.MakeIntegrationRule(4), beta_field(coef, 3))
+    x = G.fill_random(fes.num_dofs, 29)
+    y0 = G.fill_random(fes.num_dofs, 30)
+    y = _dev(y0)
+    op(_dev(x), y)
+    op.plan.set_variant(torch.float64, 0)
+    yk = _dev(y0)
+    op(_dev(x), yk)
+    op.plan.set_variant(torch.float64, -1)
+    ref = np.zeros_like(x)
+    oracle.face_operator(list(sizes), h, [0.0] * 3, 4, 0, 4, beta_numpy(coef, 3), x, ref)
+    got = y.cpu().numpy() - y0
+    assert rel_l2(got, ref) <= F64_TOL
+    assert rel_l2(got, yk.cpu().numpy() - y0) <= F64_TOL
+    assert abs(got.sum()) <= 1e-12 * np.abs(got).sum()
