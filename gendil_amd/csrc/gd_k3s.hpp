@@ -128,10 +128,21 @@ struct StreamGeom {
                                 AX[SC[2]] < 0 ? 0 : AX[SC[2]]};
   static constexpr int NTZ = (TZ[0] ? 1 : 0) + (TZ[1] ? 1 : 0) + (TZ[2] ? 1 : 0);
   static constexpr int ZC = TZ[0] ? 0 : (TZ[1] ? 1 : 2);
-  // second region: y slabs, z slabs, the in-plane tables (one padded run
-  // of TW entries each), then the tile's z-factor table
+  // second region: y slabs, z slabs, the in-plane tables, then the tile's
+  // z-factor table.  A table that varies along x takes one padded entry per
+  // element (a run of TW), one that varies along y is the same for every
+  // element of the tile's row (a single entry).
   static constexpr int NRUN = (ADV ? 2 : 0) + NTB;
-  static constexpr int ZT0 = NRUN * TW * SCS;   // first chunk of the z-factor table
+  static constexpr int ADVR = ADV ? 2 : 0;      // the slab runs
+  static constexpr bool run_shared(int r) { return r >= ADVR && SA[r - ADVR] != 0; }
+  static constexpr int run_len(int r) { return run_shared(r) ? SCH : TW * SCS; }
+  static constexpr int run0(int r)
+  {
+    int o = 0;
+    for (int k = 0; k < r; ++k) o += run_len(k);
+    return o;
+  }
+  static constexpr int ZT0 = run0(NRUN);        // first chunk of the z-factor table
   static constexpr int S_INS = (ZT0 + NTZ * SCH + 63) / 64;
   static constexpr int INS = U_INS + S_INS;     // copies per tile and lane
   static constexpr int ST = TW * CPE / 64;      // output stores per tile and lane
@@ -172,14 +183,16 @@ __host__ __device__ constexpr StreamSrc stream_src(int L, bool second)
     const int zc = L - G::ZT0;
     return zc < G::NTZ * SCH ? StreamSrc{7, zc * VW} : StreamSrc{8, 0};
   }
-  const int run = L / (TW * SCS), rr = L - run * (TW * SCS);
+  int run = 0;
+  while (run + 1 < G::NRUN && L >= G::run0(run + 1)) ++run;
+  const int rr = L - G::run0(run);
+  if (G::run_shared(run)) return rr < SCH ? StreamSrc{4 + (run - G::ADVR), rr * VW} : StreamSrc{8, 0};
   const int ee = rr / SCS, c = rr - ee * SCS;
   if (c >= SCH) return {8, 0};
   const int m = c * VW;
-  if (G::NRUN > 0 && run == 0 && (G::NRUN - G::NTB) == 2) return {2, ee * N3 + (m / N) * N2 + m % N};  // y slab (iz, ix)
-  if (G::NRUN > 0 && run == 1 && (G::NRUN - G::NTB) == 2) return {3, ee * N3 + m};                      // z slab
-  const int slot = run - (G::NRUN - G::NTB);
-  return {4 + slot, (G::SA[slot] == 0 ? ee * N2 : 0) + m};
+  if (G::ADVR == 2 && run == 0) return {2, ee * N3 + (m / N) * N2 + m % N};  // y slab (iz, ix)
+  if (G::ADVR == 2 && run == 1) return {3, ee * N3 + m};                      // z slab
+  return {4 + (run - G::ADVR), ee * N2 + m};                                 // a table varying along x
 }
 
 // Bit k set: some lane of copy instruction j takes its source from base k.
@@ -188,20 +201,7 @@ constexpr unsigned stream_kinds(int j)
 {
   unsigned m = 0;
   const bool second = j >= G::U_INS;
-  for (int lane = 0; lane < 64; ++lane) {
-    const int L = (second ? j - G::U_INS : j) * 64 + lane;
-    if (!second) {
-      const int ee = L / G::UCS, c = L - ee * G::UCS;
-      m |= 1u << ((c >= G::CPE || ee > G::TW) ? 8 : (ee == G::TW ? 1 : 0));
-    } else if (L >= G::ZT0) {
-      m |= 1u << ((L - G::ZT0) < G::NTZ * G::SCH ? 7 : 8);
-    } else {
-      const int run = L / (G::TW * G::SCS), rr = L - run * (G::TW * G::SCS);
-      const int c = rr - (rr / G::SCS) * G::SCS;
-      const int adv_runs = G::NRUN - G::NTB;
-      m |= 1u << (c >= G::SCH ? 8 : (run < adv_runs ? 2 + run : 4 + (run - adv_runs)));
-    }
-  }
+  for (int lane = 0; lane < 64; ++lane) m |= 1u << stream_src<G>((second ? j - G::U_INS : j) * 64 + lane, second).kind;
   return m;
 }
 
@@ -439,7 +439,10 @@ k3k_stream(KParams<T> p, const T* __restrict__ x, T* __restrict__ y, int ntiles)
     const T* sbase = reinterpret_cast<const T*>(cur + G::U_INS * 64);
     const T* sye = sbase + (0 * TW + el) * SCS * VW;
     const T* sze = sbase + (1 * TW + el) * SCS * VW;
-    auto ftab = [&](int d) -> const T* { return sbase + ((ADV ? 2 : 0) * TW + G::TBI[d] * TW + el) * SCS * VW; };
+    auto ftab = [&](int d) -> const T* {
+      const int r = G::ADVR + G::TBI[d];
+      return sbase + G::run0(r) * VW + (G::run_shared(r) ? 0 : el * SCS * VW);
+    };
     // this lane's row of the beta-dependent z factor
     T zt[N];
 #pragma unroll
