@@ -81,6 +81,9 @@ struct Params {
   const FieldDev<T>* mu;
   T ip_sigma;
   T ip_kappa;
+  // 1: the basis values at the element ends are unit vectors (b0 = e_0,
+  // b1 = e_{n-1}: GLL), so a face trace is the node layer at that end
+  int node_traces;
   // XCD-aware block remap
   int num_blocks;
 };

@@ -332,24 +332,34 @@ __global__ void __launch_bounds__(C::NT) dg_kernel(Params<typename C::T> p, cons
         const int a = f % D;
         const bool plus = f >= D;
         const ElemInfo<D>& inf = info[el];
-        const T* bo = plus ? b1 : b0;  // own side trace point
-        const T* bn = plus ? b0 : b1;  // neighbour side
         T own = T(0), own2 = T(0), nbv = T(0), nbv2 = T(0);
+        const long long nb = inf.e >= 0 ? inf.nb[f] : -1;
+        if (p.node_traces) {
+          // the traces are node layers: this side's at its face, the
+          // neighbour's at the opposite one
+          const int io = face_to_full<D, N, N>(t, plus ? N - 1 : 0, a);
+          own = U(el)[io];
+          if constexpr (MODE == kGradBeta) own2 = Y(el)[io];
+          if (nb >= 0) {
+            const int in = face_to_full<D, N, N>(t, plus ? 0 : N - 1, a);
+            nbv = x[nb * ND + in];
+            if constexpr (MODE == kGradBeta) nbv2 = y[nb * ND + in];
+          }
+        } else {
+          // contract the normal dimension with the basis at the face (the
+          // tables are read at uniform addresses and picked per lane: a
+          // per-lane table pointer would load them with vector loads)
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-          const T c = bo[q];
-          if (c != T(0)) {
+          for (int q = 0; q < N; ++q) {
+            const T c = plus ? b1[q] : b0[q];
             const int idx = face_to_full<D, N, N>(t, q, a);
             own += c * U(el)[idx];
             if constexpr (MODE == kGradBeta) own2 += c * Y(el)[idx];
           }
-        }
-        const long long nb = inf.e >= 0 ? inf.nb[f] : -1;
-        if (nb >= 0) {
+          if (nb >= 0) {
 #pragma unroll
-          for (int q = 0; q < N; ++q) {
-            const T c = bn[q];
-            if (c != T(0)) {
+            for (int q = 0; q < N; ++q) {
+              const T c = plus ? b0[q] : b1[q];
               const int idx = face_to_full<D, N, N>(t, q, a);
               nbv += c * x[nb * ND + idx];
               if constexpr (MODE == kGradBeta) nbv2 += c * y[nb * ND + idx];
@@ -497,8 +507,12 @@ __global__ void __launch_bounds__(C::NT) dg_kernel(Params<typename C::T> p, cons
             const bool plus = f >= D;
             int in;
             const int t = full_to_face<D, N>(idx, a, in);
-            const T c = plus ? b1[in] : b0[in];
-            if (c != T(0)) acc += c * R(el, GREG)[j * MF + t];
+            if (p.node_traces) {
+              if (in == (plus ? N - 1 : 0)) acc += R(el, GREG)[j * MF + t];
+            } else {
+              const T c = plus ? b1[in] : b0[in];
+              if (c != T(0)) acc += c * R(el, GREG)[j * MF + t];
+            }
           }
           Y(el)[idx] += acc;
         }

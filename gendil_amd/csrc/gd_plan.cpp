@@ -351,6 +351,13 @@ Params<T> make_params(const gd_plan* p, const TableSet* ts, T mass_coef, T adv_c
   k.mu = k.beta + D + 2;
   k.ip_sigma = (T)p->ip_sigma;
   k.ip_kappa = (T)p->ip_kappa;
+  {
+    const int n = ts->n;
+    bool unit = n > 0;
+    for (int i = 0; i < n; ++i)
+      unit = unit && ts->b0[i] == (i == 0 ? 1.0 : 0.0) && ts->b1[i] == (i == n - 1 ? 1.0 : 0.0);
+    k.node_traces = unit ? 1 : 0;
+  }
   k.beta_kind = p->beta_kind;
   for (int c = 0; c < D; ++c) {
     k.affc[c] = (T)p->affc[c];
