@@ -43,6 +43,10 @@ def parse():
                     help="skip the 6D p-adaptive, product-mesh, adjoint and face-operator entries")
     ap.add_argument("--no-mass", action="store_true", help="skip the 32^3 mass apply")
     ap.add_argument("--no-diffusion", action="store_true", help="skip the 64^3 SIPDG diffusion apply")
+    ap.add_argument("--no-general", action="store_true",
+                    help="skip the 64^3 applies with fields the Kronecker kernels do not take")
+    ap.add_argument("--general-only", action="store_true",
+                    help="only time the 64^3 applies with non-Kronecker fields (one JSON object)")
     ap.add_argument("--face-ops-only", action="store_true",
                     help="only time the face-centric operators (one JSON object)")
     ap.add_argument("--product6d-only", action="store_true",
@@ -588,6 +592,60 @@ def adjoint_48(steps, warmup, n=48, p=4, q=5):
     return res
 
 
+def general_64(steps, warmup, n=64):
+    """The C2 mesh and basis (3D 64^3 periodic, GLL p=3, 4 Gauss points,
+    FP64) with two fields the Kronecker kernels do not take:
+      rotating: beta = (0.5 - y, x - 0.5, 0.25), each component a function of
+        one coordinate but changing sign inside the mesh (the register
+        kernel, with its per-point upwind choice);
+      two-coordinate: beta = (1 + 0.5 sin 2pi y cos 2pi z,
+        0.5 + 0.5 cos 2pi z sin 2pi x, 0.25 + 0.25 sin 2pi x cos 2pi y), each
+        component a product over two coordinates (the generic kernel).
+    Algorithmic bytes: u read and y written once, 16 B per DoF."""
+    import numpy as np
+    import torch
+
+    import gendil_amd as G
+    from gendil_amd.fields import SeparableField, VectorField
+
+    tp = 2.0 * math.pi
+    fields = {
+        "rotating": VectorField([
+            SeparableField([(1.0, {1: lambda y: 0.5 - y})]),
+            SeparableField([(1.0, {0: lambda x: x - 0.5})]),
+            SeparableField.constant(0.25),
+        ]),
+        "two_coordinate": VectorField([
+            SeparableField([(1.0, {}), (0.5, {1: lambda y: np.sin(tp * y), 2: lambda z: np.cos(tp * z)})]),
+            SeparableField([(0.5, {}), (0.5, {2: lambda z: np.cos(tp * z), 0: lambda x: np.sin(tp * x)})]),
+            SeparableField([(0.25, {}), (0.25, {0: lambda x: np.sin(tp * x), 1: lambda y: np.cos(tp * y)})]),
+        ]),
+    }
+    mesh = G.CartesianMesh.unit(3, n, True)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    ir = G.MakeIntegrationRule(4)
+    out = {"workload": f"3D {n}^3 periodic, GLL p=3, 4 GL pts/dim, FP64, fields the Kronecker kernels do not take"}
+    x0 = None
+    for name, beta in fields.items():
+        op = G.MakeAdvectionOperator(fes, ir, beta)
+        N = op.num_dofs
+        if x0 is None:
+            x0 = torch.as_tensor(G.fill_random(N, 12345), device="cuda")
+            xs = [x0, x0 * 0.5, x0 * -0.25]
+            ys = [torch.empty_like(x0) for _ in range(3)]
+        ms = time_applies(op, xs, ys, steps, warmup)
+        out[name] = {
+            "value": N / (ms * 1e-3) / 1e9,
+            "unit": "GDoF/s",
+            "ms_per_step": ms,
+            "roofline_frac": 16.0 * N / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "variant": op.plan.query_variant(torch.float64, 0.0, 1.0),
+        }
+        del op
+    del xs, ys, x0
+    return out
+
+
 def face_ops(steps, warmup, n=64, p=3):
     """GenDiL's face-centric advection operators on device-resident vectors:
     the interior faces of a 64^3 non-periodic mesh (GLL p=3, 4 Gauss points,
@@ -679,6 +737,9 @@ def main():
     if args.product6d_only:
         print(json.dumps({"product6d": product_6d(args.steps, args.warmup)}))
         return
+    if args.general_only:
+        print(json.dumps({"general64": general_64(args.steps, args.warmup)}))
+        return
     if args.padapt_classes:
         print(json.dumps({"padapt_classes": padapt_classes(args.steps, args.warmup)}))
         return
@@ -754,6 +815,9 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_mass:
         result["mass32"] = mass_32(args.steps, args.warmup)
+        torch.cuda.empty_cache()
+    if not args.no_general:
+        result["general64"] = general_64(max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()
     if not args.no_diffusion:
         result["diffusion64"] = diffusion_64(max(10, args.steps // 4), 3)

@@ -60,6 +60,10 @@ struct K3Params {
 // axis signature: 4 digits base 4 (axis+1) for beta0, beta1, beta2, sigma
 constexpr int sig(int b0, int b1, int b2, int s) { return (b0 + 1) + 4 * (b1 + 1) + 16 * (b2 + 1) + 64 * (s + 1); }
 constexpr int sig_axis(int S, int f) { return ((S >> (2 * f)) & 3) - 1; }
+// every one-axis signature, the axes read from the kernel arguments
+constexpr int kSigAny = 256;
+// the sample axis of field f in an instance: -1..2 compiled in, 3 run time
+constexpr int field_axis(int S, int f) { return S == kSigAny ? 3 : sig_axis(S, f); }
 
 // ---- cross-lane broadcast within groups of LANES lanes -------------------
 template <int LANES, int SRC>
@@ -231,6 +235,45 @@ struct FieldSamples {
   }
 };
 
+// Run-time axis (one instance serves every one-axis signature): a constant
+// field holds its value at every sample index, so every lookup is a choice
+// among the x, y and z sample indices by the wave-uniform axis.
+template <typename T, int Q>
+struct FieldSamples<T, Q, 3> {
+  T v[Q + 2];
+  int axis;
+  __device__ __forceinline__ void load(const Field1<T>& f, const int (&sub)[3])
+  {
+    if (f.axis >= 0) {
+      axis = f.axis;
+      const int s = f.axis == 0 ? sub[0] : (f.axis == 1 ? sub[1] : sub[2]);
+      const T* tb = f.tab + s * (Q + 2);
+#pragma unroll
+      for (int k = 0; k < Q + 2; ++k) v[k] = f.scale * tb[k];
+    } else {
+      axis = 0;
+#pragma unroll
+      for (int k = 0; k < Q + 2; ++k) v[k] = f.scale;
+    }
+  }
+  __device__ __forceinline__ T at(int kx, int ky, int kz) const
+  {
+    return axis == 0 ? v[kx] : (axis == 1 ? v[ky] : v[kz]);
+  }
+  template <int LANES, int PL>
+  __device__ __forceinline__ T atc(int kx, int ky, int h, int cc) const
+  {
+    T vz = v[cc];
+#pragma unroll
+    for (int g = 1; g < LANES; ++g) vz = (h == g) ? v[g * PL + cc] : vz;
+    return axis == 0 ? v[kx] : (axis == 1 ? v[ky] : vz);
+  }
+  __device__ __forceinline__ T atzs(int kx, int ky, bool plus) const
+  {
+    return axis == 2 ? (plus ? v[Q + 1] : v[Q]) : (axis == 0 ? v[kx] : v[ky]);
+  }
+};
+
 template <typename T, int N, int LANES, bool ADV, bool MASS, bool BCG, int SIG, int NT>
 __global__ void __launch_bounds__(NT) k3d_apply(K3Params<T> p, const T* __restrict__ x, T* __restrict__ y)
 {
@@ -266,7 +309,7 @@ __global__ void __launch_bounds__(NT) k3d_apply(K3Params<T> p, const T* __restri
     return sub[a] < Ma - 1 ? e + st : (per ? e - (Ma - 1) * st : -1);
   };
 
-  constexpr int AX0 = sig_axis(SIG, 0), AX1 = sig_axis(SIG, 1), AX2 = sig_axis(SIG, 2), AXS = sig_axis(SIG, 3);
+  constexpr int AX0 = field_axis(SIG, 0), AX1 = field_axis(SIG, 1), AX2 = field_axis(SIG, 2), AXS = field_axis(SIG, 3);
   FieldSamples<T, Q, AX0> f0;
   FieldSamples<T, Q, AX1> f1;
   FieldSamples<T, Q, AX2> f2;

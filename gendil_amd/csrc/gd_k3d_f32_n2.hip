@@ -4,7 +4,7 @@
 namespace gd {
 namespace k3 {
 
-static const K3Entry<float> kList[] = {GD_K3_FOR_LANES(float, 2, 1)};
+static const K3Entry<float> kList[] = {GD_K3_FOR_LANES(float, 2, 1), GD_K3_ANY(float, 2, 1)};
 
 K3List<float> instances_f32_n2() { return K3List<float>{kList, (int)(sizeof(kList) / sizeof(kList[0]))}; }
 

@@ -81,34 +81,40 @@ bool prepare(const gd_plan* plan, int lanes, bool adv, bool mass, K3Params<T>& p
   const int want_sig = sig(ax[0], ax[1], ax[2], ax[3]);
   const bool bcg = adv && plan->bc_mode == GD_BC_INFLOW && plan->bc_host.rank > 0;
   const K3List<T> l = list_for<T>(N);
-  for (int i = 0; i < l.n; ++i) {
+  // mass-only instances ignore the beta axes
+  const int s = adv ? want_sig : sig(-1, -1, -1, ax[3]);
+  // the instance compiled for this signature, else the run-time-axis one
+  const K3Entry<T>* found = nullptr;
+  for (int i = 0; i < l.n && !found; ++i) {
     const K3Entry<T>& k = l.e[i];
-    // mass-only instances ignore the beta axes
-    const int s = adv ? want_sig : sig(-1, -1, -1, ax[3]);
-    if (k.lanes == lanes && k.adv == (int)adv && k.mass == (int)mass && k.bcg == (int)bcg && k.sig == s) {
-      *fn = k.fn;
-      prm.E = (int)plan->E;
-      prm.M0 = (int)plan->desc.sizes[0];
-      prm.M1 = (int)plan->desc.sizes[1];
-      prm.M2 = (int)plan->desc.sizes[2];
-      prm.periodic = 0;
-      double detJ = 1.0;
-      for (int d = 0; d < 3; ++d) {
-        if (plan->desc.periodic[d]) prm.periodic |= 1 << d;
-        prm.invh[d] = (T)(1.0 / plan->desc.h[d]);
-        detJ *= plan->desc.h[d];
-      }
-      prm.detJ = (T)detJ;
-      prm.bc_inflow = plan->bc_mode == GD_BC_INFLOW ? 1 : 0;
-      if constexpr (std::is_same_v<T, double>)
-        prm.bc = static_cast<const FieldDev<double>*>(plan->fields64) + 4;
-      else
-        prm.bc = static_cast<const FieldDev<float>*>(plan->fields32) + 4;
-      prm.num_blocks = 0;
-      return true;
-    }
+    if (k.lanes == lanes && k.adv == (int)adv && k.mass == (int)mass && k.bcg == (int)bcg && k.sig == s) found = &k;
   }
-  return false;
+  for (int i = 0; i < l.n && !found; ++i) {
+    const K3Entry<T>& k = l.e[i];
+    if (k.lanes == lanes && k.adv == (int)adv && k.mass == (int)mass && k.bcg == (int)bcg && k.sig == kSigAny)
+      found = &k;
+  }
+  if (!found) return false;
+  *fn = found->fn;
+  prm.E = (int)plan->E;
+  prm.M0 = (int)plan->desc.sizes[0];
+  prm.M1 = (int)plan->desc.sizes[1];
+  prm.M2 = (int)plan->desc.sizes[2];
+  prm.periodic = 0;
+  double detJ = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    if (plan->desc.periodic[d]) prm.periodic |= 1 << d;
+    prm.invh[d] = (T)(1.0 / plan->desc.h[d]);
+    detJ *= plan->desc.h[d];
+  }
+  prm.detJ = (T)detJ;
+  prm.bc_inflow = plan->bc_mode == GD_BC_INFLOW ? 1 : 0;
+  if constexpr (std::is_same_v<T, double>)
+    prm.bc = static_cast<const FieldDev<double>*>(plan->fields64) + 4;
+  else
+    prm.bc = static_cast<const FieldDev<float>*>(plan->fields32) + 4;
+  prm.num_blocks = 0;
+  return true;
 }
 
 }  // namespace

@@ -28,6 +28,13 @@ int launch(const K3Params<T>& prm_in, const T* x, T* y, hipStream_t s)
 constexpr int kSigCyclic = sig(1, 2, 0, -1);
 constexpr int kSigConst = sig(-1, -1, -1, -1);
 
+// every one-axis signature with run-time axes (the default lane count of
+// each (precision, nodes) only): advection with and without mass and boundary
+// data, and mass alone
+#define GD_K3_ANY(T, N, L)                                                                                 \
+  GD_K3_ADV(T, N, L, 0, 0, kSigAny), GD_K3_ADV(T, N, L, 1, 0, kSigAny), GD_K3_ADV(T, N, L, 0, 1, kSigAny), \
+      GD_K3_ADV(T, N, L, 1, 1, kSigAny), K3Entry<T>{N, L, 0, 1, 0, kSigAny, &launch<T, N, L, false, true, false, kSigAny>}
+
 #define GD_K3_ADV(T, N, L, MASS, BCG, SIG) \
   K3Entry<T> { N, L, 1, MASS, BCG, SIG, &launch<T, N, L, true, MASS, BCG, SIG> }
 #define GD_K3_FOR_LANES(T, N, L)                                                                            \

@@ -235,3 +235,58 @@ def test_streaming_kernel_needs_whole_tiles():
     op(torch.as_tensor(x, device="cuda"), y)
     ref = oracle.Problem(fes, G.MakeIntegrationRule(4), adv_coef=1.0, adv=W.advection_c2()).apply(x)
     assert rel_l2(y.cpu().numpy(), ref) <= 1e-12
+
+
+def one_axis_field(kind):
+    """One-axis fields whose axis signatures have no compiled instance of the
+    register kernel: the run-time-axis instance serves them."""
+    from gendil_amd.fields import SeparableField, VectorField
+    if kind == "rotating":  # (0.5 - y, x - 0.5, 0.25): signs change inside the mesh
+        return VectorField([SeparableField([(1.0, {1: lambda s: 0.5 - s})]),
+                            SeparableField([(1.0, {0: lambda s: s - 0.5})]), SeparableField.constant(0.25)])
+    if kind == "own_axis":  # (sin 2 pi x, -0.5, 0.3 + cos 2 pi z): beta_d varies along d
+        return VectorField([SeparableField([(1.0, {0: lambda s: np.sin(2 * np.pi * s)})]),
+                            SeparableField.constant(-0.5),
+                            SeparableField([(1.0, {2: lambda s: 0.3 + np.cos(2 * np.pi * s)})])])
+    raise ValueError(kind)
+
+
+RUNTIME_AXIS_CASES = [
+    # (p, n, periodic, kind, sigma y-dependent, bc, mass_coef, adv_coef, variants)
+    (3, 5, True, "rotating", False, None, 0.0, 1.0, (2, 4)),
+    (3, 4, False, "own_axis", True, "poly", 0.7, 1.0, (2, 4)),
+    (3, 4, False, "rotating", False, "zero", 0.0, 1.0, (2, 4)),
+    (2, 5, False, "own_axis", False, "zero", 1.0, 0.5, (1,)),
+    (1, 6, True, "rotating", True, None, 0.3, 1.0, (1,)),
+]
+
+
+@pytest.mark.parametrize("case", RUNTIME_AXIS_CASES,
+                         ids=[f"p{c[0]}_n{c[1]}_{c[3]}_bc{c[5]}_m{c[6]}" for c in RUNTIME_AXIS_CASES])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_register_kernel_runtime_axes_vs_oracle(case, dtype):
+    """Fields with any one-axis signature (rotating, varying along their own
+    axis, mixed signs) run on the register kernel, not the generic one, and
+    match the oracle."""
+    from gendil_amd.fields import SeparableField
+    p, n, periodic, kind, sig_y, bc, mc, ac, lanes_list = case
+    mesh = G.CartesianMesh.unit(3, n, periodic)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(p))
+    ir = G.MakeIntegrationRule(p + 1)
+    beta = one_axis_field(kind)
+    sigma = SeparableField([(1.0, {1: lambda s: 1.0 + s})]) if sig_y else None
+    g = bc_field(bc, 3) if bc else None
+    op = G.MakeMassAdvectionOperator(fes, ir, beta, sigma=sigma, bc=g, mass_coef=mc, adv_coef=ac)
+    x = G.fill_random(op.num_dofs, 3)
+    ref = oracle.Problem(fes, ir, mass_coef=mc, adv_coef=ac, sigma=sigma, adv=beta, bc=g).apply(x)
+    if dtype == torch.float32:
+        lanes_list = (1,) if p < 3 else (4,)
+    for lanes in (-1, *lanes_list):
+        op.plan.set_variant(dtype, lanes)
+        v = op.plan.query_variant(dtype, mc, ac)
+        print(f"runtime axes {case} {dtype} lanes={lanes} -> variant {v}", flush=True)
+        assert v in (1, 2, 4), v
+        y = torch.empty(op.num_dofs, dtype=dtype, device="cuda")
+        op(torch.as_tensor(x, dtype=dtype, device="cuda"), y)
+        err = rel_l2(y.double().cpu().numpy(), ref)
+        assert err <= (1e-12 if dtype == torch.float64 else 1e-5), (lanes, err)
