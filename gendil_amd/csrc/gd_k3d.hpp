@@ -237,7 +237,10 @@ struct FieldSamples {
 
 // Run-time axis (one instance serves every one-axis signature): a constant
 // field holds its value at every sample index, so every lookup is a choice
-// among the x, y and z sample indices by the wave-uniform axis.
+// among the x, y and z sample indices by the wave-uniform axis.  The
+// candidates are pinned in registers before the choice (a select of elements
+// of one array would become a load at a run-time index, putting the samples
+// on the stack).
 template <typename T, int Q>
 struct FieldSamples<T, Q, 3> {
   T v[Q + 2];
@@ -256,21 +259,29 @@ struct FieldSamples<T, Q, 3> {
       for (int k = 0; k < Q + 2; ++k) v[k] = f.scale;
     }
   }
-  __device__ __forceinline__ T at(int kx, int ky, int kz) const
+  __device__ __forceinline__ T choose(T a, T b, T c) const
   {
-    return axis == 0 ? v[kx] : (axis == 1 ? v[ky] : v[kz]);
+    __asm__ volatile("" : "+v"(a), "+v"(b), "+v"(c));
+    return axis == 0 ? a : (axis == 1 ? b : c);
   }
+  __device__ __forceinline__ T at(int kx, int ky, int kz) const { return choose(v[kx], v[ky], v[kz]); }
   template <int LANES, int PL>
   __device__ __forceinline__ T atc(int kx, int ky, int h, int cc) const
   {
     T vz = v[cc];
 #pragma unroll
-    for (int g = 1; g < LANES; ++g) vz = (h == g) ? v[g * PL + cc] : vz;
-    return axis == 0 ? v[kx] : (axis == 1 ? v[ky] : vz);
+    for (int g = 1; g < LANES; ++g) {
+      T o = v[g * PL + cc];
+      __asm__ volatile("" : "+v"(vz), "+v"(o));
+      vz = (h == g) ? o : vz;
+    }
+    return choose(v[kx], v[ky], vz);
   }
   __device__ __forceinline__ T atzs(int kx, int ky, bool plus) const
   {
-    return axis == 2 ? (plus ? v[Q + 1] : v[Q]) : (axis == 0 ? v[kx] : v[ky]);
+    T lo = v[Q], hi = v[Q + 1];
+    __asm__ volatile("" : "+v"(lo), "+v"(hi));
+    return choose(v[kx], v[ky], plus ? hi : lo);
   }
 };
 
@@ -617,10 +628,19 @@ __global__ void __launch_bounds__(NT) k3d_apply(K3Params<T> p, const T* __restri
           for (int a = 0; a < Q; ++a) {
             const T ext = boundary ? gval(a, b, kn) : nbt[b * Q + a];
             const T d = dot[b * Q + a];
-            // own trace: boundary plane of the x-y interpolation (select, not
-            // a runtime register index, when the plane depends on the lane)
-            const T ownv = (PL == 1 || LANES == 1) ? Txy[kk_f][b * Q + a]
-                                                   : (s ? Txy[PL - 1][b * Q + a] : Txy[0][b * Q + a]);
+            // own trace: boundary plane of the x-y interpolation.  When the
+            // plane depends on the lane both candidates are first pinned in
+            // registers: a select of two elements of one array would
+            // otherwise become one load at a lane-dependent index, which puts
+            // the whole array on the stack.
+            T ownv;
+            if constexpr (PL == 1 || LANES == 1) {
+              ownv = Txy[kk_f][b * Q + a];
+            } else {
+              T lo = Txy[0][b * Q + a], hi = Txy[PL - 1][b * Q + a];
+              __asm__ volatile("" : "+v"(lo), "+v"(hi));
+              ownv = s ? hi : lo;
+            }
             const T up = d > T(0) ? d * ownv : d * ext;
             F[a] = active ? p.adv_coef * ((T)(TB.w[a] * TB.w[b]) * detJ * up) : T(0);
           }
