@@ -236,52 +236,45 @@ struct FieldSamples {
 };
 
 // Run-time axis (one instance serves every one-axis signature): a constant
-// field holds its value at every sample index, so every lookup is a choice
-// among the x, y and z sample indices by the wave-uniform axis.  The
-// candidates are pinned in registers before the choice (a select of elements
-// of one array would become a load at a run-time index, putting the samples
-// on the stack).
+// field holds its value at every sample index, and a lookup is the one-hot
+// weighted sum wx v[kx] + wy v[ky] + wz v[kz] with the wave-uniform weights of
+// the field's axis.  Products with 0 and 1 and sums with 0 are exact for the
+// finite samples, and unlike a select between elements of one array (which
+// the optimiser turns into a load at a run-time index, putting the samples on
+// the stack) the sum leaves every sample in a register.
 template <typename T, int Q>
 struct FieldSamples<T, Q, 3> {
   T v[Q + 2];
-  int axis;
+  T wx, wy, wz;
   __device__ __forceinline__ void load(const Field1<T>& f, const int (&sub)[3])
   {
+    wx = f.axis <= 0 ? T(1) : T(0);  // a constant is read at the x index
+    wy = f.axis == 1 ? T(1) : T(0);
+    wz = f.axis == 2 ? T(1) : T(0);
     if (f.axis >= 0) {
-      axis = f.axis;
       const int s = f.axis == 0 ? sub[0] : (f.axis == 1 ? sub[1] : sub[2]);
       const T* tb = f.tab + s * (Q + 2);
 #pragma unroll
       for (int k = 0; k < Q + 2; ++k) v[k] = f.scale * tb[k];
     } else {
-      axis = 0;
 #pragma unroll
       for (int k = 0; k < Q + 2; ++k) v[k] = f.scale;
     }
   }
-  __device__ __forceinline__ T choose(T a, T b, T c) const
-  {
-    __asm__ volatile("" : "+v"(a), "+v"(b), "+v"(c));
-    return axis == 0 ? a : (axis == 1 ? b : c);
-  }
-  __device__ __forceinline__ T at(int kx, int ky, int kz) const { return choose(v[kx], v[ky], v[kz]); }
+  __device__ __forceinline__ T at(int kx, int ky, int kz) const { return wx * v[kx] + wy * v[ky] + wz * v[kz]; }
   template <int LANES, int PL>
   __device__ __forceinline__ T atc(int kx, int ky, int h, int cc) const
   {
-    T vz = v[cc];
+    // z sample h * PL + cc: the lane's own, again as a one-hot sum
+    T vz = T(0);
 #pragma unroll
-    for (int g = 1; g < LANES; ++g) {
-      T o = v[g * PL + cc];
-      __asm__ volatile("" : "+v"(vz), "+v"(o));
-      vz = (h == g) ? o : vz;
-    }
-    return choose(v[kx], v[ky], vz);
+    for (int g = 0; g < LANES; ++g) vz += (h == g ? T(1) : T(0)) * v[g * PL + cc];
+    return wx * v[kx] + wy * v[ky] + wz * vz;
   }
   __device__ __forceinline__ T atzs(int kx, int ky, bool plus) const
   {
-    T lo = v[Q], hi = v[Q + 1];
-    __asm__ volatile("" : "+v"(lo), "+v"(hi));
-    return choose(v[kx], v[ky], plus ? hi : lo);
+    const T vz = (plus ? T(0) : T(1)) * v[Q] + (plus ? T(1) : T(0)) * v[Q + 1];
+    return wx * v[kx] + wy * v[ky] + wz * vz;
   }
 };
 
