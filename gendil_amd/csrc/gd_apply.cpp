@@ -76,8 +76,9 @@ int k3_choice(const gd_plan* plan, int N, bool adv, bool mass)
   if (ks >= 0 && k3k::available<T>(plan, ks, adv, mass)) return 16 + ks;
   const int kl = k3k::default_lanes(N, sizeof(T));
   if (k3k::available<T>(plan, kl, adv, mass)) return 16 + kl;
-  // measured on MI355X for the 64^3 p=3 workload: FP64 2 lanes, FP32 4 lanes
-  if (N == 4) return sizeof(T) == 8 ? 2 : 4;
+  // measured on MI355X at 64^3 p=3 (a one-axis field the Kronecker kernels
+  // do not take): 2 lanes 171 us FP64 / 88 us FP32, 4 lanes 209 / 94 us
+  if (N == 4) return 2;
   return 1;
 }
 
@@ -141,7 +142,7 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
         if (r != GD_ERR_UNSUPPORTED) return r;
       }
     }
-    const int lanes = choice >= 16 ? (N == 4 ? (sizeof(T) == 8 ? 2 : 4) : 1) : choice;
+    const int lanes = choice >= 16 ? (N == 4 ? 2 : 1) : choice;
     if (lanes > 0) {
       const int r = k3::apply<T>(plan, lanes, mass_coef, adv_coef, x, y, s);
       if (r != GD_ERR_UNSUPPORTED) return r;
@@ -296,7 +297,7 @@ int gd_plan_query_variant(const gd_plan* plan, int32_t precision_bytes, double m
       return gd::n2k::kVariant;
     const int choice = gd::k3_choice<T>(plan, N, adv, mass);
     if (choice >= 16 && gd::k3k::available<T>(plan, choice - 16, adv, mass)) return choice;
-    const int lanes = choice >= 16 ? (N == 4 ? (sizeof(T) == 8 ? 2 : 4) : 1) : choice;
+    const int lanes = choice >= 16 ? (N == 4 ? 2 : 1) : choice;
     if (lanes > 0 && gd::k3::available<T>(plan, lanes, adv, mass)) return lanes;
     return 0;
   };

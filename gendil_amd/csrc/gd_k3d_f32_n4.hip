@@ -4,7 +4,7 @@
 namespace gd {
 namespace k3 {
 
-static const K3Entry<float> kList[] = {GD_K3_FOR_LANES(float, 4, 1), GD_K3_FOR_LANES(float, 4, 2), GD_K3_FOR_LANES(float, 4, 4), GD_K3_ANY(float, 4, 4)};
+static const K3Entry<float> kList[] = {GD_K3_FOR_LANES(float, 4, 1), GD_K3_FOR_LANES(float, 4, 2), GD_K3_FOR_LANES(float, 4, 4), GD_K3_ANY(float, 4, 2), GD_K3_ANY(float, 4, 4)};
 
 K3List<float> instances_f32_n4() { return K3List<float>{kList, (int)(sizeof(kList) / sizeof(kList[0]))}; }
 

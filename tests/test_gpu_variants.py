@@ -280,7 +280,7 @@ def test_register_kernel_runtime_axes_vs_oracle(case, dtype):
     x = G.fill_random(op.num_dofs, 3)
     ref = oracle.Problem(fes, ir, mass_coef=mc, adv_coef=ac, sigma=sigma, adv=beta, bc=g).apply(x)
     if dtype == torch.float32:
-        lanes_list = (1,) if p < 3 else (4,)
+        lanes_list = (1,) if p < 3 else (2, 4)
     for lanes in (-1, *lanes_list):
         op.plan.set_variant(dtype, lanes)
         v = op.plan.query_variant(dtype, mc, ac)
