@@ -129,8 +129,6 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
     const int r = n2k::apply<T>(plan, mass_coef, adv_coef, x, y, s);
     if (r != GD_ERR_UNSUPPORTED) return r;
   }
-  const TableSet* ts = find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
-  const Params<T> prm = make_params<T>(plan, ts, mass_coef, adv_coef);
   {
     const int N = plan->desc.order + 1;
     const int choice = k3_choice<T>(plan, N, adv, mass_coef != T(0));
@@ -148,6 +146,10 @@ int apply_impl(const gd_plan* plan, T mass_coef, T adv_coef, const T* x, T* y, h
       if (r != GD_ERR_UNSUPPORTED) return r;
     }
   }
+  // the generic kernel (its parameter block is built only here: the kernels
+  // above take their own)
+  const TableSet* ts = find_tables(plan, plan->desc.order + 1, plan->desc.num_quad);
+  const Params<T> prm = make_params<T>(plan, ts, mass_coef, adv_coef);
   if constexpr (std::is_same_v<T, double>) return adv ? ks->fwd64_adv(prm, x, y, s) : ks->fwd64_mass(prm, x, y, s);
   else return adv ? ks->fwd32_adv(prm, x, y, s) : ks->fwd32_mass(prm, x, y, s);
 }

@@ -22,7 +22,16 @@ from .discretization import FiniteElementSpace, IntegrationRule, make_desc
 from .fields import AffineField, PointwiseField, SeparableField, VectorField, face_counts
 
 
+_get_device = getattr(torch._C, "_cuda_getDevice", None)
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream():
+    """The current HIP stream of the current device (torch's current stream),
+    read without building a torch Stream object: applies of small meshes are
+    submitted in a few microseconds, and that object costs about as much."""
+    if _get_device is not None and _raw_stream is not None:
+        return C.c_void_p(_raw_stream(_get_device()))
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
@@ -41,18 +50,20 @@ def _overlap(a: torch.Tensor, b: torch.Tensor) -> bool:
     return a0 < b1 and b0 < a1
 
 
-def _span(t, fes, name):
-    """The space's part of `t`: all of it, or the num_dofs entries from the
-    space's L2Restriction shift when `t` is a vector shared by several spaces
-    (MakeReadOnlyElementTensorView on a shifted space, elementtensorview.hpp:42-64)."""
-    n = fes.num_dofs
-    if fes.shift == 0 and isinstance(t, torch.Tensor) and t.numel() == n:
+def _span(t, plan, name):
+    """The space's part of `t`: all of it, or the plan's num_dofs entries from
+    its space's L2Restriction shift when `t` is a vector shared by several
+    spaces (MakeReadOnlyElementTensorView on a shifted space,
+    elementtensorview.hpp:42-64).  Sizes come from the plan (fixed when it was
+    created), not recomputed from the space on every apply."""
+    n, shift = plan.num_dofs, plan.shift
+    if shift == 0 and isinstance(t, torch.Tensor) and t.numel() == n:
         return t
     if not isinstance(t, torch.Tensor) or t.dim() != 1:
         raise TypeError(f"{name} must be a 1-D torch tensor")
-    if fes.shift < 0 or fes.shift + n > t.numel():
-        raise ValueError(f"{name}: {t.numel()} entries cannot hold {n} DoFs from shift {fes.shift}")
-    return t.narrow(0, fes.shift, n)
+    if shift < 0 or shift + n > t.numel():
+        raise ValueError(f"{name}: {t.numel()} entries cannot hold {n} DoFs from shift {shift}")
+    return t.narrow(0, shift, n)
 
 
 class Plan:
@@ -68,6 +79,7 @@ class Plan:
         self.handle = h
         self._keep = [keep]
         self.num_dofs = int(L.lib().gd_plan_num_dofs(self.handle))
+        self.shift = int(fes.shift)
         self.beta_kind = None
 
     def __del__(self):
@@ -191,7 +203,7 @@ class Plan:
     # ---- applies ----
     def apply(self, mass_coef: float, adv_coef: float, x: torch.Tensor, y: torch.Tensor):
         out = y
-        x, y = _span(x, self.fes, "x"), _span(y, self.fes, "y")
+        x, y = _span(x, self, "x"), _span(y, self, "y")
         if x.dtype == torch.float64:
             _check_vec(x, self.num_dofs, torch.float64, "x")
             _check_vec(y, self.num_dofs, torch.float64, "y")
@@ -350,7 +362,7 @@ class AdvectionFaceOperator:
         return self.plan.num_dofs
 
     def __call__(self, x: torch.Tensor, y: torch.Tensor):
-        xs, ys = _span(x, self.plan.fes, "x"), _span(y, self.plan.fes, "y")
+        xs, ys = _span(x, self.plan, "x"), _span(y, self.plan, "y")
         _check_vec(xs, self.plan.num_dofs, torch.float64, "x")
         _check_vec(ys, self.plan.num_dofs, torch.float64, "y")
         if _overlap(xs, ys):
@@ -373,11 +385,10 @@ class AdvectionNonconformingFaceOperator:
         self._entry = "gd_adv_hmesh_apply_f64" if refined else "gd_adv_intermesh_apply_f64"
 
     def __call__(self, x: torch.Tensor, y: torch.Tensor):
-        fm, fp = self.minus.fes, self.plus.fes
-        xm, xp = _span(x, fm, "x"), _span(x, fp, "x")
-        ym, yp = _span(y, fm, "y"), _span(y, fp, "y")
-        for t, n, name in ((xm, fm.num_dofs, "x"), (xp, fp.num_dofs, "x"), (ym, fm.num_dofs, "y"),
-                           (yp, fp.num_dofs, "y")):
+        nm, np_ = self.minus.num_dofs, self.plus.num_dofs
+        xm, xp = _span(x, self.minus, "x"), _span(x, self.plus, "x")
+        ym, yp = _span(y, self.minus, "y"), _span(y, self.plus, "y")
+        for t, n, name in ((xm, nm, "x"), (xp, np_, "x"), (ym, nm, "y"), (yp, np_, "y")):
             _check_vec(t, n, torch.float64, name)
         # the kernels read x while they add into y (the h-refined interface in
         # one launch per subface position), so no part of x may share storage
@@ -407,7 +418,7 @@ class DiffusionOperator:
 
     def __call__(self, x: torch.Tensor, y: torch.Tensor):
         out = y
-        x, y = _span(x, self.plan.fes, "x"), _span(y, self.plan.fes, "y")
+        x, y = _span(x, self.plan, "x"), _span(y, self.plan, "y")
         if x.dtype not in (torch.float64, torch.float32):
             raise TypeError("x must be float64 or float32")
         _check_vec(x, self.num_dofs, x.dtype, "x")
