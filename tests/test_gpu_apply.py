@@ -157,3 +157,34 @@ def test_grad_beta_matches_oracle_and_fd():
     op(to_dev(u), y)
     yo = oracle.Problem(fes, ir, adv_coef=1.0, adv=(cell, faces)).apply(u)
     assert rel_l2(y.cpu().numpy(), yo) <= F64_TOL
+
+
+def test_apply_runs_on_the_current_torch_stream():
+    """The apply launches on torch's current stream, read through the raw
+    stream query: it matches torch.cuda.current_stream() on the default
+    stream and inside a side stream, and an apply queued on a side stream
+    behind a slow producer sees the producer's values."""
+    from gendil_amd import operators as O
+    # (a null stream handle reads back as None from a c_void_p)
+    assert (O._stream().value or 0) == torch.cuda.current_stream().cuda_stream
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        assert (O._stream().value or 0) == side.cuda_stream != 0
+    mesh = G.CartesianMesh.unit(3, 8, True)
+    fes = G.MakeFiniteElementSpace(mesh, G.MakeLobattoFiniteElement(3))
+    ir = G.MakeIntegrationRule(4)
+    op = G.MakeAdvectionOperator(fes, ir, W.advection_c2())
+    x = G.fill_random(op.num_dofs, 11)
+    ref = oracle.Problem(fes, ir, adv_coef=1.0, adv=W.advection_c2()).apply(x)
+    xd = torch.as_tensor(x, device="cuda")
+    big = torch.rand(1 << 24, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        xs = torch.zeros_like(xd)
+        for _ in range(4):  # a slow producer ahead of the copy on the same stream
+            big.mul_(1.0000001)
+        xs.copy_(xd)
+        y = torch.empty_like(xd)
+        op(xs, y)
+    side.synchronize()
+    assert rel_l2(y.cpu().numpy(), ref) <= 1e-12
