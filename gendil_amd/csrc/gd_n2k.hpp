@@ -84,21 +84,13 @@ __device__ __forceinline__ void sfor(F&& f)
   }
 }
 
+// One element (or the lanes past the last element, which load in bounds and
+// store nothing): e is this lane's element, node its node.
 template <typename T, int D>
-__global__ void __launch_bounds__(256) n2k_kernel(N2Params<T> p, const T* __restrict__ x, T* __restrict__ y)
+__device__ __forceinline__ void apply_one(const N2Params<T>& p, const T* tab, const T* __restrict__ x,
+                                          T* __restrict__ y, int e, int node)
 {
-  constexpr int ND = 1 << D, EPW = 64 / ND;
-  // the coefficient table in LDS: each lane reads its own rows there (held in
-  // scalar registers the 77 values would not fit)
-  __shared__ T tab[N2Params<T>::kTab];
-  for (int i = (int)threadIdx.x; i < N2Params<T>::kTab; i += (int)blockDim.x) tab[i] = p.tab[i];
-  __syncthreads();
-  const int lane = (int)threadIdx.x & 63, node = lane & (ND - 1);
-  // (blocks in launch order: one contiguous run of elements per XCD halves
-  // the fetched bytes on a 10^6 mesh but measured 15 % slower there, and
-  // equal at 5^6)
-  int e = ((int)blockIdx.x * ((int)blockDim.x / 64) + (int)threadIdx.x / 64) * EPW + lane / ND;
-  if constexpr (EPW == 1) e = __builtin_amdgcn_readfirstlane(e);  // one element per wave: scalar index work
+  constexpr int ND = 1 << D;
   const bool valid = e < p.E;
   const int ec = valid ? e : p.E - 1;  // lanes past the last element load in bounds, store nothing
   const T u = x[(long long)ec * ND + node];
@@ -145,10 +137,48 @@ __global__ void __launch_bounds__(256) n2k_kernel(N2Params<T> p, const T* __rest
 }
 
 template <typename T, int D>
+__global__ void __launch_bounds__(256) n2k_kernel(N2Params<T> p, const T* __restrict__ x, T* __restrict__ y)
+{
+  constexpr int ND = 1 << D, EPW = 64 / ND;
+  // the coefficient table in LDS: each lane reads its own rows there (held in
+  // scalar registers the 77 values would not fit)
+  __shared__ T tab[N2Params<T>::kTab];
+  for (int i = (int)threadIdx.x; i < N2Params<T>::kTab; i += (int)blockDim.x) tab[i] = p.tab[i];
+  __syncthreads();
+  const int lane = (int)threadIdx.x & 63, node = lane & (ND - 1);
+  // Blocks stay resident and stride over the mesh (the table is loaded once
+  // per block, not once per four elements); within a round, blocks in launch
+  // order.  (One contiguous run of elements per XCD halves the fetched bytes
+  // on a 10^6 mesh but measured 15 % slower there, and equal at 5^6.)
+  // the wave's first element: wave-uniform, so every lane of a wave runs the
+  // same iterations (the lane exchanges need the whole wave)
+  const int wave_e = ((int)blockIdx.x * ((int)blockDim.x / 64) + (int)threadIdx.x / 64) * EPW;
+  const int estride = (int)gridDim.x * ((int)blockDim.x / 64) * EPW;
+  for (int w0 = wave_e; w0 < p.E; w0 += estride) {
+    int e = w0 + lane / ND;
+    if constexpr (EPW == 1) e = __builtin_amdgcn_readfirstlane(e);  // one element per wave: scalar index work
+    apply_one<T, D>(p, tab, x, y, e, node);
+  }
+}
+
+template <typename T, int D>
 int launch(const N2Params<T>& prm, const T* x, T* y, hipStream_t s)
 {
   constexpr int EPB = 4 * (64 >> D);  // four waves per block
-  const int nb = (prm.E + EPB - 1) / EPB;
+  // at most the blocks the device holds at once (queried once per process:
+  // CUs x resident blocks of this instance); the kernel strides over the
+  // rest, so no block starts late with a full share of the mesh
+  static int resident = 0;
+  if (resident == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, n2k_kernel<T, D>, 256, 0) != hipSuccess)
+      return -1;
+    resident = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+  }
+  const int need = (prm.E + EPB - 1) / EPB;
+  const int nb = need < resident ? need : resident;
   if (nb == 0) return 0;
   hipLaunchKernelGGL((n2k_kernel<T, D>), dim3((unsigned)nb), dim3(256), 0, s, prm, x, y);
   return hipGetLastError() == hipSuccess ? 0 : -1;
