@@ -242,9 +242,13 @@ int main()
       // BiCGSTAB's iterates follow the rounding of its inner products (host
       // sums here, fixed-order device reductions there), so the two runs drift
       // apart over hundreds of iterations; both stop at a relative residual
-      // of 1e-8, which bounds their difference by that times K's condition
+      // of 1e-8, which bounds their difference by that times K's condition.
+      // GenDiL's own count is not reproducible on this system: repeated runs
+      // of this binary on 16 OpenMP threads stopped after 477 and 515
+      // iterations, while the binding's device reductions stop after 456
+      // every time, so the counts are held to within a fifth of each other.
       const long di = (long)r_ref.iterations - (long)r_mi.iterations;
-      const long slack = std::max<long>(2, (long)r_ref.iterations / 10);
+      const long slack = std::max<long>(2, (long)r_ref.iterations / 5);
       if (!r_ref.success || !r_mi.success || di > slack || di < -slack) ++failures;
       report("BiCGSTAB_no_alloc: binding vs GenDiL iterate", rel_l2(x_mi, x_ref, 0, nd), 1e-5);
     }
