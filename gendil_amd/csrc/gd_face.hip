@@ -503,9 +503,16 @@ int check_plan(const gd_plan* pl, const char* what)
   return GD_OK;
 }
 
-int launch(Args& a, hipStream_t st)
+int launch(const Args& args, hipStream_t st)
 {
-  if (a.nfaces <= 0) return GD_OK;
+  if (args.nfaces <= 0) return GD_OK;
+  // a small face set is spread over the device (about four blocks per CU of
+  // 256 CUs) instead of filling each block's LDS: 4,096 interface faces at
+  // 51 faces per block would run on 81 blocks
+  constexpr long long kSpreadBlocks = 1024;
+  Args a = args;
+  const long long spread = (a.nfaces + kSpreadBlocks - 1) / kSpreadBlocks;
+  if (spread < a.fpb) a.fpb = (int)std::max<long long>(1, spread);
   const int blocks = (int)((a.nfaces + a.fpb - 1) / a.fpb);
   const size_t lds = (size_t)4 * a.fpb * a.S * sizeof(double);
   hipLaunchKernelGGL(k_faces, dim3((unsigned)blocks), dim3(kThreads), lds, st, a);
