@@ -244,12 +244,12 @@ int main()
       // apart over hundreds of iterations; both stop at a relative residual
       // of 1e-8, which bounds their difference by that times K's condition.
       // GenDiL's own count is not reproducible on this system: repeated runs
-      // of this binary on 16 OpenMP threads stopped after 477 and 515
+      // of this binary on 16 OpenMP threads stopped after 477, 515 and 614
       // iterations, while the binding's device reductions stop after 456
-      // every time, so the counts are held to within a fifth of each other.
-      const long di = (long)r_ref.iterations - (long)r_mi.iterations;
+      // every time.  So both must converge, agree in the iterate, and the
+      // binding may not need more than a fifth more iterations than GenDiL.
       const long slack = std::max<long>(2, (long)r_ref.iterations / 5);
-      if (!r_ref.success || !r_mi.success || di > slack || di < -slack) ++failures;
+      if (!r_ref.success || !r_mi.success || (long)r_mi.iterations > (long)r_ref.iterations + slack) ++failures;
       report("BiCGSTAB_no_alloc: binding vs GenDiL iterate", rel_l2(x_mi, x_ref, 0, nd), 1e-5);
     }
   }
