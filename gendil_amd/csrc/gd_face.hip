@@ -595,14 +595,17 @@ extern "C" int gd_adv_face_apply_f64(const gd_plan* plan, const double* x, doubl
     const long long per = (4LL * a.S + nd) * (long long)sizeof(double);
     const int fpb = (int)std::min<long long>(kMaxFpb, kOwnerLdsBytes / per);
     if (fpb >= 8 && n <= kMaxN && n * a.q <= kMaxQN) {
-      // one launch: every element adds its own share of each of its faces
-      a.fpb = fpb;
+      // one launch: every element adds its own share of each of its faces;
+      // a small mesh is spread over about 1,024 blocks as in launch()
+      const long long spread = (plan->E + 1023) / 1024;
+      const int use = (int)std::max<long long>(1, std::min<long long>(fpb, spread));
+      a.fpb = use;
       a.nfaces = plan->E;
       if (int rc = side_setup(plan, a.q, 0, x, y, a.s[0])) return rc;
       if (int rc = side_setup(plan, a.q, 1, x, y, a.s[1])) return rc;
       if (a.nfaces <= 0) return GD_OK;
-      const int blocks = (int)((a.nfaces + fpb - 1) / fpb);
-      hipLaunchKernelGGL(owner_kernel(D, n, a.q), dim3((unsigned)blocks), dim3(kThreads), (size_t)fpb * per, st, a);
+      const int blocks = (int)((a.nfaces + use - 1) / use);
+      hipLaunchKernelGGL(owner_kernel(D, n, a.q), dim3((unsigned)blocks), dim3(kThreads), (size_t)use * per, st, a);
       const hipError_t e = hipGetLastError();
       return e == hipSuccess ? GD_OK : gd::hip_fail(e, "face kernel launch");
     }
